@@ -1,0 +1,140 @@
+// fsm_internal.h -- structures shared by the host planner and the HIP kernels.
+//
+// Device data layout (HBM, all structure-of-arrays):
+//   * 2-D spline planes: float4 per knot {z, z_y*hy, z_x*hx, z_xy*hx*hy},
+//     x-major (knot index ix*NY + iy), z = converted density e0/(2 pi).
+//   * 3-D source planes: float per cell, eta-major (ieta*NX*NY + ix*NY + iy),
+//     holding e0 * norm_factor (the cell-independent factor of
+//     convertInitialDensity, src/FreeStream.cpp:302-303,346).
+//   * T^{mu nu}: 10 float planes.  Reference layout is (ix*NY + iy)*NETA +
+//     ieta; the 3-D stream kernel writes an eta-major layout
+//     (ieta*NX*NY + ix*NY + iy) which the Landau kernel reads directly.
+//   * Outputs: 17 double planes in output_to_vectors order.
+#ifndef FSM_INTERNAL_H_
+#define FSM_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FSM_HBARC_F 0.197326938f
+
+// ---------------------------------------------------------------- 2-D ----
+// Cells per lane along x in the 2-D stream kernel; one wave covers 64
+// consecutive iy.  Tile = (4 waves * FSM2D_M) x 64 cells.
+#define FSM2D_M 4
+#define FSM2D_WAVES 4
+#define FSM2D_TX (FSM2D_WAVES * FSM2D_M)
+#define FSM2D_TY 64
+#define FSM2D_THREADS (FSM2D_WAVES * 64)
+
+// Band half-width of the natural-spline derivative operator.  The operator
+// is the inverse of the tridiagonal (1,4,1) spline system times a central
+// difference; its entries decay as (2 - sqrt 3)^|i-m| = 0.268^|i-m|, so
+// entries beyond 24 are below 2e-14 of the diagonal (checked on the host).
+#define FSM_SPLINE_BAND 24
+#define FSM_SPLINE_W (2 * FSM_SPLINE_BAND + 1)
+#define FSMSP_TX 16
+#define FSMSP_TY 64
+
+// One azimuthal sample of the 2-D stencil.  For a fixed phi_j the source
+// point x - DTAU cos(phi_j) is the same shift for every cell
+// (src/FreeStream.cpp:184-189), so the floor offset and the interpolation
+// weights are per-j constants.
+struct Stencil2D {
+  int ox, oy;       // knot offset of the lower-left corner, in cells
+  float hx[4];      // Hermite weights in x: value0, slope0, value1, slope1
+  float hy[4];      // Hermite weights in y
+  float tx, ty;     // bilinear fractions
+  float c, s;       // cos(phi_j), sin(phi_j) as the reference computes them
+  float cc, cs, ss; // hypertrig products c*c, c*s, s*s
+  float dxj, dyj;   // cos(phi_j)*DTAU, sin(phi_j)*DTAU (float, as the reference)
+  float pad0, pad1, pad2;
+};
+
+// A contiguous range of azimuthal samples whose stencil offsets share a
+// bounding box; the 2-D kernel stages that box of knots in LDS once per range.
+struct StencilGroup {
+  int j0, j1;  // [j0, j1)
+  int bx0, bx1, by0, by1;  // knot offset box: ox in [bx0, bx1-1], knots up to bx1
+  int pad0, pad1;
+};
+
+struct Event2D {
+  int nx, ny;
+  int ngroups;
+  int lds_floats4;      // LDS float4 elements needed by the largest group
+  float out_scale;      // DIM_RAP (identical rapidity copies in 2-D)
+  float d_phip;         // (float)(2 pi / DIM_PHIP)
+  float tau;            // TAU
+  int tiles_x;          // tiles along x
+  float xmin, ymin;     // grid origin as getX/getY compute it
+  float dx, dy;
+  const float *e_gev;   // input, GeV/fm^3
+  float4 *planes;       // spline planes (written by the spline kernel)
+  const float *gx;      // [nx][FSM_SPLINE_W] derivative band weights along x
+  const float *gy;      // [ny][FSM_SPLINE_W]
+  const Stencil2D *st;  // [DIM_PHIP]
+  const StencilGroup *groups;
+  float *T;             // 10 planes, reference layout
+  long t_plane;         // plane stride of T (>= DIM)
+};
+
+// ---------------------------------------------------------------- 3-D ----
+#define FSM3D_THREADS 256
+#define FSM3D_M 4
+#define FSM3D_TILE (FSM3D_THREADS * FSM3D_M)
+
+// Per (ieta, irap) quantities of the 3-D path.  The source rapidity plane,
+// the eta interpolation fraction and the transverse shift magnitude D depend
+// on (ieta, irap) only (src/FreeStream.cpp:174-195).
+struct RapSlot {
+  int valid;        // eta-mask passes and D is finite
+  int eta0;         // floor of the source eta index
+  float feta;       // eta fraction
+  float D;          // TAU cosh(rap-eta_new) - TAU0 cosh(rap-eta)
+  float rf0, rf1;   // rapidity factor of convertInitialDensity at planes eta0, eta0+1
+  float jac;        // rapidity Jacobian of calculateStressTensor
+  float inv_ch;     // 1/cosh(rap-eta)
+  float inv_ch2;    // 1/cosh^2(rap-eta)
+  float th_t;       // tanh(rap-eta)/TAU
+  float th_tch;     // tanh(rap-eta)/(TAU cosh(rap-eta))
+  float th2_t2;     // tanh^2(rap-eta)/TAU^2
+  int rmin, rmax;   // row offset range touched by this slot (min/max floor offset in x)
+  int pad0, pad1;
+};
+
+struct Event3D {
+  int nx, ny, neta, nrap, nphi;
+  int tiles;            // transverse tiles per eta plane
+  int lds_floats;       // LDS floats available for the staged plane
+  float d_phip;
+  float inv_dx, inv_dy; // 1/DX, 1/DY: shift in cells = -(cos(phi) * D) * inv_dx
+  float dx, dy;
+  float xmin, ymin;
+  const float *e_gev;   // input GeV/fm^3, reference layout
+  float *src;           // eta-major e0*norm planes
+  const RapSlot *slots; // [neta][nrap]
+  const float *cs;      // [2*nphi]: cos(phi_j), sin(phi_j)
+  float *T;             // 10 planes, eta-major layout
+  long t_plane;
+  float norm_factor;
+  float pad;
+};
+
+// --------------------------------------------------------------- Landau ----
+struct EventLandau {
+  int dim;
+  int nxy, neta;        // for the eta-major T layout
+  int t_eta_major;      // 1: T is eta-major (3-D stream output)
+  int eos_type;
+  int viscous;
+  int fill_outputs;     // OUTPUTFORMAT is 2 or 3
+  float tau;
+  const float *T;
+  long t_plane;
+  double *out;          // 17 planes of dim doubles (may be null)
+  long out_plane;
+  float *fe, *fu, *fp, *fPi, *fpi;  // optional float outputs
+};
+
+#endif  // FSM_INTERNAL_H_

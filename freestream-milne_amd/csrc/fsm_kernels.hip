@@ -1,0 +1,689 @@
+// fsm_kernels.hip -- CDNA4 (gfx950) kernels of the free-streaming +
+// Landau-matching step.
+//
+//   k_spline2d   ingest (src/FreestreamMilne.cpp:267) + convertInitialDensity
+//                in 2-D (src/FreeStream.cpp:308-322) + the knot derivatives
+//                of the natural bicubic spline (src/FreeStream.cpp:121-153),
+//                one LDS tile per work item.                       HBM-bound
+//   k_stream2d   freeStream 2-D (src/FreeStream.cpp:156-248) fused with
+//                calculateHypertrigTable / calculateStressTensor
+//                (src/LandauMatch.cpp:14-108): per azimuth sample, the
+//                bicubic value (bilinear when the spline is <= 0) of the knot
+//                planes staged in LDS, accumulated into six angular moments.
+//                                                              FP32 VALU-bound
+//   k_src3d      ingest + convertInitialDensity norm in 3-D, transposed to
+//                eta-major planes.                                 HBM-bound
+//   k_stream3d   freeStream 3-D (src/FreeStream.cpp:190-273) + quadrature:
+//                per (eta, rapidity slot) the eta-blended source plane is
+//                staged in LDS and each azimuth sample is a bilinear gather.
+//                                                              FP32 VALU-bound
+//   k_landau     solveEigenSystem (src/LandauMatch.cpp:145-331) by FP64
+//                hyperbolic Jacobi rotations, calculatePressure
+//                (src/EquationOfState.cpp:5-80), calculateBulkPressure and
+//                calculateShearViscTensor (src/LandauMatch.cpp:332-367) and
+//                the output pack (src/FreestreamMilne.cpp:695-718).  HBM-bound
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+
+#include "fsm_internal.h"
+#include "fsm_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ float ingest_e0(float v) {
+  // src/FreestreamMilne.cpp:267: v * rescale / (float)hbarc + lower_tolerance
+  return v * 1.0f / FSM_HBARC_F + 1.0e-7f;
+}
+
+// Mask decision of the reference for a sample whose shifted index lies next
+// to the mask boundary (src/FreeStream.cpp:163,187,198,204,213), evaluated
+// with the reference's own float operations and rounding:
+//   x = (float)i*D + xmin;  x_new = x - shift;  f = floor((x_new - xmin)/D)
+// and kept iff f in [1, n-2].  Away from the boundary the per-phi stencil
+// offset decides, since a one-ulp difference cannot move the floor across it.
+__device__ __forceinline__ bool ref_mask_ok(int i, float shift, float mn, float h, int n) {
+  const float x = __fadd_rn(__fmul_rn((float)i, h), mn);
+  const float xn = __fsub_rn(x, shift);
+  const float f = floorf(__fdiv_rn(__fsub_rn(xn, mn), h));
+  return (f >= 1.0f) && (f < (float)(n - 1));
+}
+
+__device__ __forceinline__ bool mask_ok(int i, int g, float shift, float mn, float h, int n) {
+  if ((unsigned)g <= 1u || (unsigned)(g - (n - 2)) <= 1u) return ref_mask_ok(i, shift, mn, h, n);
+  return (unsigned)(g - 1) <= (unsigned)(n - 3);
+}
+
+// ======================================================================
+// 2-D spline knot planes
+// ======================================================================
+#define SP_RX (FSMSP_TX + 2 * FSM_SPLINE_BAND)
+#define SP_RY (FSMSP_TY + 2 * FSM_SPLINE_BAND)
+
+__device__ void spline2d_tile(const Event2D &ev, int tx, int ty, float *zr, float *zxr) {
+  const int nx = ev.nx, ny = ev.ny;
+  const int x0 = tx * FSMSP_TX, y0 = ty * FSMSP_TY;
+  const int B = FSM_SPLINE_BAND;
+  for (int k = threadIdx.x; k < SP_RX * SP_RY; k += blockDim.x) {
+    int lx = k / SP_RY, ly = k - lx * SP_RY;
+    int gx = x0 - B + lx, gy = y0 - B + ly;
+    float z = 0.0f;
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+      float e0 = ingest_e0(ev.e_gev[(long)gx * ny + gy]);
+      z = (float)((double)e0 / (2.0 * M_PI));  // src/FreeStream.cpp:318
+    }
+    zr[k] = z;
+  }
+  __syncthreads();
+  // z_x (in grid units) for the tile rows and the y-halo.
+  for (int k = threadIdx.x; k < FSMSP_TX * SP_RY; k += blockDim.x) {
+    int lx = k / SP_RY, ly = k - lx * SP_RY;
+    int gx = x0 + lx, gy = y0 - B + ly;
+    float acc = 0.0f;
+    if (gx < nx && gy >= 0 && gy < ny) {
+      const float *w = ev.gx + (long)gx * FSM_SPLINE_W;
+#pragma unroll 7
+      for (int m = 0; m < FSM_SPLINE_W; ++m) acc = fmaf(w[m], zr[(lx + m) * SP_RY + ly], acc);
+    }
+    zxr[k] = acc;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < FSMSP_TX * FSMSP_TY; k += blockDim.x) {
+    int lx = k / FSMSP_TY, ly = k - lx * FSMSP_TY;
+    int gx = x0 + lx, gy = y0 + ly;
+    if (gx < nx && gy < ny) {
+      const float *w = ev.gy + (long)gy * FSM_SPLINE_W;
+      float zy = 0.0f, zxy = 0.0f;
+#pragma unroll 7
+      for (int m = 0; m < FSM_SPLINE_W; ++m) {
+        zy = fmaf(w[m], zr[(lx + B) * SP_RY + ly + m], zy);
+        zxy = fmaf(w[m], zxr[lx * SP_RY + ly + m], zxy);
+      }
+      float z = zr[(lx + B) * SP_RY + ly + B];
+      float zx = zxr[lx * SP_RY + ly + B];
+      ev.planes[(long)gx * ny + gy] = make_float4(z, zy, zx, zxy);
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_spline2d_one(Event2D ev) {
+  __shared__ float zr[SP_RX * SP_RY];
+  __shared__ float zxr[FSMSP_TX * SP_RY];
+  int tiles_y = (ev.ny + FSMSP_TY - 1) / FSMSP_TY;
+  spline2d_tile(ev, blockIdx.x / tiles_y, blockIdx.x % tiles_y, zr, zxr);
+}
+
+__global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restrict__ evs,
+                                                        const int4 *__restrict__ items, int n_items) {
+  __shared__ float zr[SP_RX * SP_RY];
+  __shared__ float zxr[FSMSP_TX * SP_RY];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    int4 item = items[it];
+    const Event2D ev = evs[item.x];
+    spline2d_tile(ev, item.y, item.z, zr, zxr);
+  }
+}
+
+// ======================================================================
+// 2-D streaming + quadrature
+// ======================================================================
+__device__ void stream2d_tile(const Event2D &ev, int tile_x, int tile_y, float4 *lds) {
+  const int nx = ev.nx, ny = ev.ny;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = tile_x * FSM2D_TX, y0 = tile_y * FSM2D_TY;
+  const int iy = y0 + lane;
+  const int ixb = x0 + wave * FSM2D_M;
+  float acc[FSM2D_M][6];
+#pragma unroll
+  for (int m = 0; m < FSM2D_M; ++m)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] = 0.0f;
+
+  for (int g = 0; g < ev.ngroups; ++g) {
+    const StencilGroup G = ev.groups[g];
+    const int LX = FSM2D_TX + G.bx1 - G.bx0;
+    const int LY = FSM2D_TY + G.by1 - G.by0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < LX * LY; k += blockDim.x) {
+      int lx = k / LY, ly = k - lx * LY;
+      int gx = x0 + G.bx0 + lx, gy = y0 + G.by0 + ly;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = ev.planes[(long)gx * ny + gy];
+      lds[k] = v;
+    }
+    __syncthreads();
+    for (int j = G.j0; j < G.j1; ++j) {
+      const Stencil2D S = ev.st[j];
+      // LDS index of knot (a, b) = (ix + a + ox, iy + b + oy) relative to the box
+      const int base = (wave * FSM2D_M + S.ox - G.bx0) * LY + (lane + S.oy - G.by0);
+      const bool yok = mask_ok(iy, iy + S.oy, S.dyj, ev.ymin, ev.dy, ny);
+      float Zy[FSM2D_M + 1], ZXy[FSM2D_M + 1], Bl[FSM2D_M + 1];
+#pragma unroll
+      for (int a = 0; a <= FSM2D_M; ++a) {
+        const float4 k0 = lds[base + a * LY];
+        const float4 k1 = lds[base + a * LY + 1];
+        Zy[a] = fmaf(S.hy[0], k0.x, fmaf(S.hy[1], k0.y, fmaf(S.hy[2], k1.x, S.hy[3] * k1.y)));
+        ZXy[a] = fmaf(S.hy[0], k0.z, fmaf(S.hy[1], k0.w, fmaf(S.hy[2], k1.z, S.hy[3] * k1.w)));
+        Bl[a] = fmaf(S.ty, k1.x - k0.x, k0.x);
+      }
+#pragma unroll
+      for (int m = 0; m < FSM2D_M; ++m) {
+        float sp = fmaf(S.hx[0], Zy[m], fmaf(S.hx[1], ZXy[m], fmaf(S.hx[2], Zy[m + 1], S.hx[3] * ZXy[m + 1])));
+        float bl = fmaf(S.tx, Bl[m + 1] - Bl[m], Bl[m]);
+        // src/FreeStream.cpp:245-246: spline value if > 0, else bilinear
+        float v = sp > 0.0f ? sp : bl;
+        // src/FreeStream.cpp:213: floor index in [1, N-2] in x and y
+        bool ok = yok && mask_ok(ixb + m, ixb + m + S.ox, S.dxj, ev.xmin, ev.dx, nx);
+        v = ok ? v : 0.0f;
+        acc[m][0] += v;
+        acc[m][1] = fmaf(v, S.c, acc[m][1]);
+        acc[m][2] = fmaf(v, S.s, acc[m][2]);
+        acc[m][3] = fmaf(v, S.cc, acc[m][3]);
+        acc[m][4] = fmaf(v, S.cs, acc[m][4]);
+        acc[m][5] = fmaf(v, S.ss, acc[m][5]);
+      }
+    }
+  }
+  // src/LandauMatch.cpp:104: T = sum * d_phip / TAU.  2-D hypertrig rows 3, 6,
+  // 8, 9 are tanh(0) multiples and vanish.
+  const long P = ev.t_plane;
+#pragma unroll
+  for (int m = 0; m < FSM2D_M; ++m) {
+    int ix = ixb + m;
+    if (ix < nx && iy < ny) {
+      long is = (long)ix * ny + iy;
+      float r[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) r[c] = ((acc[m][c] * ev.out_scale) * ev.d_phip) / ev.tau;
+      ev.T[0 * P + is] = r[0];
+      ev.T[1 * P + is] = r[1];
+      ev.T[2 * P + is] = r[2];
+      ev.T[3 * P + is] = 0.0f;
+      ev.T[4 * P + is] = r[3];
+      ev.T[5 * P + is] = r[4];
+      ev.T[6 * P + is] = 0.0f;
+      ev.T[7 * P + is] = r[5];
+      ev.T[8 * P + is] = 0.0f;
+      ev.T[9 * P + is] = 0.0f;
+    }
+  }
+}
+
+__global__ __launch_bounds__(FSM2D_THREADS) void k_stream2d_one(Event2D ev) {
+  extern __shared__ float4 lds4[];
+  stream2d_tile(ev, blockIdx.x % ev.tiles_x, blockIdx.x / ev.tiles_x, lds4);
+}
+
+__global__ __launch_bounds__(FSM2D_THREADS) void k_stream2d_batch(const Event2D *__restrict__ evs,
+                                                                  const int4 *__restrict__ items,
+                                                                  int n_items) {
+  extern __shared__ float4 lds4[];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    int4 item = items[it];
+    const Event2D ev = evs[item.x];
+    stream2d_tile(ev, item.y, item.z, lds4);
+  }
+}
+
+// ======================================================================
+// 3-D source planes and streaming
+// ======================================================================
+// e_gev (reference layout [nxy][neta]) -> src (eta-major [neta][nxy]) with
+// src = e0 * norm_factor (src/FreeStream.cpp:346, first product).
+__global__ __launch_bounds__(256) void k_src3d(const float *__restrict__ e_gev, float *__restrict__ src,
+                                               int nxy, int neta, float norm_factor) {
+  __shared__ float tile[64][65];
+  const int xy0 = blockIdx.x * 64, et0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, tyb = threadIdx.x >> 6;
+  for (int r = tyb; r < 64; r += 4) {
+    int xy = xy0 + r, et = et0 + tx;
+    float v = 0.0f;
+    if (xy < nxy && et < neta) v = ingest_e0(e_gev[(long)xy * neta + et]) * norm_factor;
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  for (int r = tyb; r < 64; r += 4) {
+    int et = et0 + r, xy = xy0 + tx;
+    if (xy < nxy && et < neta) src[(long)et * nxy + xy] = tile[tx][r];
+  }
+}
+
+// Generic plane transpose: in [planes][rows][cols] -> out [planes][cols][rows].
+__global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ in, float *__restrict__ out,
+                                                   int rows, int cols, long in_plane, long out_plane) {
+  __shared__ float tile[64][65];
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, p = blockIdx.z;
+  const int tx = threadIdx.x & 63, tyb = threadIdx.x >> 6;
+  const float *ip = in + p * in_plane;
+  float *op = out + p * out_plane;
+  for (int r = tyb; r < 64; r += 4) {
+    int rr = r0 + r, cc = c0 + tx;
+    tile[r][tx] = (rr < rows && cc < cols) ? ip[(long)rr * cols + cc] : 0.0f;
+  }
+  __syncthreads();
+  for (int r = tyb; r < 64; r += 4) {
+    int cc = c0 + r, rr = r0 + tx;
+    if (rr < rows && cc < cols) op[(long)cc * rows + rr] = tile[tx][r];
+  }
+}
+
+// Q(x, y) = (1-f) rf0 E[eta0] + f rf1 E[eta0+1], zero outside the grid.
+__device__ __forceinline__ float q_global(const float *p0, const float *p1, float w0, float w1, float r0f,
+                                          float r1f, int x, int y, int nx, int ny) {
+  if (x < 0 || x >= nx || y < 0 || y >= ny) return 0.0f;
+  const long a = (long)x * ny + y;
+  return w0 * (p0[a] * r0f) + w1 * (p1[a] * r1f);
+}
+
+// One transverse tile (FSM3D_TILE consecutive cells of the ix*NY+iy order)
+// of one eta plane.  For each valid rapidity slot the eta-blended source plane
+// rows the tile can reach are staged in LDS with row pitch NY+1 and one zero
+// row/column of halo, so a sample that the boundary rule keeps with a
+// neighbouring floor still reads in-bounds (its value is continuous there).
+__device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
+  const int nx = ev.nx, ny = ev.ny, nxy = nx * ny;
+  const int pitch = ny + 1;
+  const int c0 = tile * FSM3D_TILE;
+  const int row_lo = c0 / ny;
+  const int row_hi = min(c0 + FSM3D_TILE - 1, nxy - 1) / ny;
+  int cx[FSM3D_M], cy[FSM3D_M];
+  bool live[FSM3D_M];
+#pragma unroll
+  for (int m = 0; m < FSM3D_M; ++m) {
+    int c = c0 + threadIdx.x + FSM3D_THREADS * m;
+    live[m] = c < nxy;
+    cx[m] = c / ny;
+    cy[m] = c - cx[m] * ny;
+  }
+  float T[FSM3D_M][10];
+#pragma unroll
+  for (int m = 0; m < FSM3D_M; ++m)
+#pragma unroll
+    for (int v = 0; v < 10; ++v) T[m][v] = 0.0f;
+
+  for (int k = 0; k < ev.nrap; ++k) {
+    const RapSlot S = ev.slots[(long)ieta * ev.nrap + k];
+    if (!S.valid) continue;
+    // rows [r0, r1] (r0 may be -1, r1 may be nx: zero rows)
+    const int r0 = max(-1, row_lo + S.rmin - 1);
+    const int r1 = min(nx, row_hi + S.rmax + 2);
+    if (r0 > r1) continue;
+    const int nrows = r1 - r0 + 1;
+    const bool use_lds = (long)nrows * pitch <= (long)ev.lds_floats;
+    const float w0 = 1.0f - S.feta, w1 = S.feta;
+    const float *p0 = ev.src + (long)S.eta0 * nxy;
+    const float *p1 = ev.src + (long)(S.eta0 + 1) * nxy;
+    __syncthreads();
+    if (use_lds) {
+      for (int e = threadIdx.x; e < nrows * pitch; e += blockDim.x) {
+        const int r = e / pitch, y = e - r * pitch;
+        q[e] = q_global(p0, p1, w0, w1, S.rf0, S.rf1, r0 + r, y, nx, ny);
+      }
+    }
+    __syncthreads();
+    float mo[FSM3D_M][6];
+#pragma unroll
+    for (int m = 0; m < FSM3D_M; ++m)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) mo[m][c] = 0.0f;
+    for (int j = 0; j < ev.nphi; ++j) {
+      const float c = ev.cs[2 * j], s = ev.cs[2 * j + 1];
+      // src/FreeStream.cpp:193-194: x_new = x - cos(phip) * D
+      const float dxj = c * S.D, dyj = s * S.D;
+      const float sx = -dxj * ev.inv_dx;
+      const float sy = -dyj * ev.inv_dy;
+      const float fx = floorf(sx), fy = floorf(sy);
+      const int ox = (int)fx, oy = (int)fy;
+      const float tx = sx - fx, ty = sy - fy;
+      // a sample is kept only if its floor index is in [1, nx-2], which
+      // needs ix+ox in [0, nx-1]
+      if (row_hi + ox < 0 || row_lo + ox > nx - 1) continue;
+      const float cc = c * c, csv = c * s, ss = s * s;
+#pragma unroll
+      for (int m = 0; m < FSM3D_M; ++m) {
+        const int sxi = cx[m] + ox, syi = cy[m] + oy;
+        const bool ok = live[m] && mask_ok(cx[m], sxi, dxj, ev.xmin, ev.dx, nx) &&
+                        mask_ok(cy[m], syi, dyj, ev.ymin, ev.dy, ny);
+        float v = 0.0f;
+        if (ok) {
+          float q00, q01, q10, q11;
+          if (use_lds) {
+            const int a = (sxi - r0) * pitch + syi;
+            q00 = q[a];
+            q01 = q[a + 1];
+            q10 = q[a + pitch];
+            q11 = q[a + pitch + 1];
+          } else {
+            q00 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi, syi, nx, ny);
+            q01 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi, syi + 1, nx, ny);
+            q10 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi + 1, syi, nx, ny);
+            q11 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi + 1, syi + 1, nx, ny);
+          }
+          const float l0 = fmaf(ty, q01 - q00, q00);
+          const float l1 = fmaf(ty, q11 - q10, q10);
+          v = fmaf(tx, l1 - l0, l0);
+        }
+        mo[m][0] += v;
+        mo[m][1] = fmaf(v, c, mo[m][1]);
+        mo[m][2] = fmaf(v, s, mo[m][2]);
+        mo[m][3] = fmaf(v, cc, mo[m][3]);
+        mo[m][4] = fmaf(v, csv, mo[m][4]);
+        mo[m][5] = fmaf(v, ss, mo[m][5]);
+      }
+    }
+    // hypertrig rows (src/LandauMatch.cpp:46-55) times the Jacobian (:93,101)
+#pragma unroll
+    for (int m = 0; m < FSM3D_M; ++m) {
+      const float j0 = S.jac;
+      T[m][0] = fmaf(j0, mo[m][0], T[m][0]);
+      T[m][1] = fmaf(j0 * S.inv_ch, mo[m][1], T[m][1]);
+      T[m][2] = fmaf(j0 * S.inv_ch, mo[m][2], T[m][2]);
+      T[m][3] = fmaf(j0 * S.th_t, mo[m][0], T[m][3]);
+      T[m][4] = fmaf(j0 * S.inv_ch2, mo[m][3], T[m][4]);
+      T[m][5] = fmaf(j0 * S.inv_ch2, mo[m][4], T[m][5]);
+      T[m][6] = fmaf(j0 * S.th_tch, mo[m][1], T[m][6]);
+      T[m][7] = fmaf(j0 * S.inv_ch2, mo[m][5], T[m][7]);
+      T[m][8] = fmaf(j0 * S.th_tch, mo[m][2], T[m][8]);
+      T[m][9] = fmaf(j0 * S.th2_t2, mo[m][0], T[m][9]);
+    }
+  }
+  const long P = ev.t_plane;
+#pragma unroll
+  for (int m = 0; m < FSM3D_M; ++m) {
+    if (!live[m]) continue;
+    const long idx = (long)ieta * nxy + cx[m] * ny + cy[m];
+#pragma unroll
+    for (int v = 0; v < 10; ++v) ev.T[v * P + idx] = T[m][v] * ev.d_phip;  // :105
+  }
+}
+
+__global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_one(Event3D ev) {
+  extern __shared__ float qlds[];
+  stream3d_tile(ev, blockIdx.x % ev.tiles, blockIdx.x / ev.tiles, qlds);
+}
+
+__global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D *__restrict__ evs,
+                                                                  const int4 *__restrict__ items,
+                                                                  int n_items) {
+  extern __shared__ float qlds[];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    int4 item = items[it];
+    const Event3D ev = evs[item.x];
+    stream3d_tile(ev, item.y, item.z, qlds);
+  }
+}
+
+// ======================================================================
+// Landau matching + EoS + viscous parts + output pack
+// ======================================================================
+// Landau frame of one cell.  In the orthonormal frame (eta components scaled
+// by TAU) the matrix T^mu_nu = T^{mu rho} g_{rho nu} of
+// src/LandauMatch.cpp:177-214 becomes A eta with A symmetric; its timelike
+// eigenvector is the frame in which A^{0i} = 0.  Each step is the exact
+// Lorentz boost in the (0,i) plane that zeroes A^{0i}:
+//   tanh(2 theta) = -2 A^{0i} / (A^{00} + A^{ii}),
+// i.e. the minimiser of u.A.u along that boost, which is the closed form
+// e = (a-c)/2 + sqrt((a+c)^2/4 - b^2) of the 2x2 block.  Cyclic sweeps over
+// i = x, y, eta converge to the Landau frame; the product of the inverse
+// boosts maps (1,0,0,0) to u.  Selection rules of the reference
+// (src/LandauMatch.cpp:228-291): e > 0, u timelike, u^tau < 100, else
+// e = 1e-3 and u = (1,0,0,0).
+__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  const double tau = (double)TAU;
+  double A[4][4], M[4][4];
+  A[0][0] = t[0];
+  A[0][1] = A[1][0] = t[1];
+  A[0][2] = A[2][0] = t[2];
+  A[0][3] = A[3][0] = tau * (double)t[3];
+  A[1][1] = t[4];
+  A[1][2] = A[2][1] = t[5];
+  A[1][3] = A[3][1] = tau * (double)t[6];
+  A[2][2] = t[7];
+  A[2][3] = A[3][2] = tau * (double)t[8];
+  A[3][3] = tau * tau * (double)t[9];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) M[i][j] = (i == j) ? 1.0 : 0.0;
+  bool ok = true, conv = false;
+  for (int sweep = 0; sweep < 60 && ok; ++sweep) {
+    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
+    if (off <= 1.0e-28 * A[0][0] * A[0][0]) {
+      conv = true;
+      break;
+    }
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+      const double b = A[0][i];
+      if (b == 0.0) continue;
+      const double s = A[0][0] + A[i][i];
+      const double disc = (s - 2.0 * fabs(b)) * (s + 2.0 * fabs(b));
+      if (!(s > 0.0) || !(disc > 0.0)) {
+        ok = false;
+        break;
+      }
+      const double rq = rsqrt(disc);
+      const double ch2 = s * rq;          // cosh 2theta
+      const double sh2 = -2.0 * b * rq;   // sinh 2theta
+      const double ch = sqrt(0.5 * (ch2 + 1.0));
+      const double sh = sh2 / (2.0 * ch);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double a0 = A[0][k], ai = A[i][k];
+        A[0][k] = ch * a0 + sh * ai;
+        A[i][k] = sh * a0 + ch * ai;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double a0 = A[k][0], ai = A[k][i];
+        A[k][0] = ch * a0 + sh * ai;
+        A[k][i] = sh * a0 + ch * ai;
+      }
+      A[0][i] = A[i][0] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double m0 = M[k][0], mi = M[k][i];
+        M[k][0] = ch * m0 - sh * mi;
+        M[k][i] = ch * mi - sh * m0;
+      }
+    }
+  }
+  if (ok && !conv) {
+    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
+    conv = off <= 1.0e-12 * A[0][0] * A[0][0];
+  }
+  const double e = A[0][0];
+  const double u0 = M[0][0];
+  if (ok && conv && e > 0.0 && u0 < 100.0 && isfinite(u0)) {
+    *e_out = (float)e;
+    u_out[0] = (float)u0;
+    u_out[1] = (float)M[1][0];
+    u_out[2] = (float)M[2][0];
+    u_out[3] = (float)(M[3][0] / tau);
+  } else {
+    *e_out = 1.0e-3f;
+    u_out[0] = 1.0f;
+    u_out[1] = u_out[2] = u_out[3] = 0.0f;
+  }
+}
+
+__device__ float eos_pressure(float e, int eos) {
+  if (eos == 1) return (float)((double)e / 3.0);
+  if (eos == 2) {
+    const float e1 = e, e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e, e6 = e5 * e, e7 = e6 * e,
+                e8 = e7 * e, e9 = e8 * e, e10 = e9 * e, e11 = e10 * e, e12 = e11 * e;
+    const float a =
+        fmaf(1.9531729608963267e-11f, e12,
+             fmaf(3.1188455176941583e-7f, e11,
+                  fmaf(0.0009417586777847889f, e10,
+                       fmaf(0.7158279081255019f, e9,
+                            fmaf(141.5073484468774f, e8,
+                                 fmaf(6340.448389300905f, e7,
+                                      fmaf(41913.439282708554f, e6,
+                                           fmaf(334334.4309240126f, e5,
+                                                fmaf(1.6357487344679043e6f, e4,
+                                                     fmaf(3.1729694865420084e6f, e3,
+                                                          fmaf(1.077580993288114e6f, e2,
+                                                               fmaf(9737.845799644809f, e1,
+                                                                    -0.25181736420168666f))))))))))));
+    const float b =
+        fmaf(3.2581066229887368e-18f, e12,
+             fmaf(5.928138360995685e-11f, e11,
+                  fmaf(9.601103399348206e-7f, e10,
+                       fmaf(0.002962497695527404f, e9,
+                            fmaf(2.3405487982094204f, e8,
+                                 fmaf(499.04919730607065f, e7,
+                                      fmaf(26452.34905933697f, e6,
+                                           fmaf(278581.2989342773f, e5,
+                                                fmaf(1.7851642641834426e6f, e4,
+                                                     fmaf(1.3512402226067686e7f, e3,
+                                                          fmaf(2.0931169138134286e7f, e2,
+                                                               fmaf(4.0574329080826794e6f, e1,
+                                                                    45829.44617893836f))))))))))));
+    return a / b;
+  }
+  return 0.0f;
+}
+
+__device__ void landau_cell_out(const EventLandau &ev, long is) {
+#pragma clang fp contract(off)
+  long tidx = is;
+  if (ev.t_eta_major) {
+    const long xy = is / ev.neta;
+    const long ie = is - xy * ev.neta;
+    tidx = ie * ev.nxy + xy;
+  }
+  float t[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) t[c] = ev.T[c * ev.t_plane + tidx];
+  float e, u[4];
+  landau_fp64(t, ev.tau, &e, u);
+  const float TAU = ev.tau;
+  const float p = eos_pressure(e, ev.eos_type);
+  float Pi = 0.0f, pi[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) pi[c] = 0.0f;
+  if (ev.viscous) {
+    const float a = t[0] - t[4] - t[7] - TAU * TAU * t[9];
+    Pi = (float)((-1.0 / 3.0) * (double)(a - e) - (double)p);
+    const float b = e + p + Pi;
+    const float c = p + Pi;
+    pi[0] = t[0] - u[0] * u[0] * b + c;
+    pi[1] = t[1] - u[0] * u[1] * b;
+    pi[2] = t[2] - u[0] * u[2] * b;
+    pi[3] = t[3] - u[0] * u[3] * b;
+    pi[4] = t[4] - u[1] * u[1] * b - c;
+    pi[5] = t[5] - u[1] * u[2] * b;
+    pi[6] = t[6] - u[1] * u[3] * b;
+    pi[7] = t[7] - u[2] * u[2] * b - c;
+    pi[8] = t[8] - u[2] * u[3] * b;
+    pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * (1.0 / (double)(TAU * TAU)));
+  }
+  if (ev.out) {
+    const long P = ev.out_plane;
+    double *o = ev.out;
+    if (ev.fill_outputs) {
+      const double hb = (double)FSM_HBARC_F;
+      o[0 * P + is] = (double)e * hb;
+      o[1 * P + is] = (double)p * hb;
+      o[2 * P + is] = (double)u[0];
+      o[3 * P + is] = (double)u[1];
+      o[4 * P + is] = (double)u[2];
+      o[5 * P + is] = (double)u[3];
+#pragma unroll
+      for (int c = 0; c < 10; ++c) o[(6 + c) * P + is] = (double)pi[c] * hb;
+      o[16 * P + is] = (double)Pi * hb;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 17; ++c) o[c * P + is] = 0.0;
+    }
+  }
+  const long D = ev.dim;
+  if (ev.fe) ev.fe[is] = e;
+  if (ev.fu) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ev.fu[c * D + is] = u[c];
+  }
+  if (ev.fp) ev.fp[is] = p;
+  if (ev.fPi) ev.fPi[is] = Pi;
+  if (ev.fpi) {
+#pragma unroll
+    for (int c = 0; c < 10; ++c) ev.fpi[c * D + is] = pi[c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long is = (long)blockIdx.x * blockDim.x + threadIdx.x; is < ev.dim; is += stride)
+    landau_cell_out(ev, is);
+}
+
+// Batched: item = (event, first cell, cell count).
+__global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restrict__ evs,
+                                                      const int4 *__restrict__ items, int n_items) {
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    int4 item = items[it];
+    const EventLandau ev = evs[item.x];
+    for (int c = threadIdx.x; c < item.z; c += blockDim.x) landau_cell_out(ev, (long)item.y + c);
+  }
+}
+
+}  // namespace
+
+// ======================================================================
+// Launchers (host)
+// ======================================================================
+hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s) {
+  int tiles = ((ev.nx + FSMSP_TX - 1) / FSMSP_TX) * ((ev.ny + FSMSP_TY - 1) / FSMSP_TY);
+  hipLaunchKernelGGL(k_spline2d_one, dim3(tiles), dim3(256), 0, s, ev);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_stream2d(const Event2D &ev, hipStream_t s) {
+  int tiles_y = (ev.ny + FSM2D_TY - 1) / FSM2D_TY;
+  size_t lds = (size_t)ev.lds_floats4 * sizeof(float4);
+  hipLaunchKernelGGL(k_stream2d_one, dim3(ev.tiles_x * tiles_y), dim3(FSM2D_THREADS), lds, s, ev);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_src3d(const Event3D &ev, hipStream_t s) {
+  int nxy = ev.nx * ev.ny;
+  dim3 grid((nxy + 63) / 64, (ev.neta + 63) / 64);
+  hipLaunchKernelGGL(k_src3d, grid, dim3(256), 0, s, ev.e_gev, ev.src, nxy, ev.neta, ev.norm_factor);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_stream3d(const Event3D &ev, hipStream_t s) {
+  size_t lds = (size_t)ev.lds_floats * sizeof(float);
+  hipLaunchKernelGGL(k_stream3d_one, dim3(ev.tiles * ev.neta), dim3(FSM3D_THREADS), lds, s, ev);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_transpose(const float *in, float *out, int rows, int cols, int planes, long in_plane,
+                                long out_plane, hipStream_t s) {
+  dim3 grid((rows + 63) / 64, (cols + 63) / 64, planes);
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, in, out, rows, cols, in_plane, out_plane);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s) {
+  int blocks = (int)std::min<long>((ev.dim + 255) / 256, 65535L);
+  hipLaunchKernelGGL(k_landau_one, dim3(blocks), dim3(256), 0, s, ev);
+  return hipGetLastError();
+}
+
+hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_stream2d_one,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes2d);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_stream2d_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          bytes2d);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_stream3d_one, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          bytes3d);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_stream3d_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             bytes3d);
+}

@@ -1,0 +1,332 @@
+"""Python host mirror of the MI355X free-streaming + Landau-matching library.
+
+Thin ctypes layer over ``libfsm.so`` (the C ABI declared in
+``include/fsm.h``).  It mirrors the reference's operator interface for this
+path -- class ``FREESTREAMMILNE`` of src/FreestreamMilne.cpp:28-87 with
+``initialize_from_vector``, ``run_freestream_milne``, ``output_to_vectors``,
+``tau_LandauMatch`` and the ``final_*`` vectors -- and exposes the
+device-resident entry points for callers that keep events in HBM.
+
+The library is required: importing this module raises if ``libfsm.so`` has
+not been built (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+try:  # load torch's HIP runtime first so one runtime serves both libraries
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover - torch is optional for host-buffer use
+    torch = None
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_PKG, "libfsm.so")
+
+OUTPUT_NAMES = (
+    "e", "p", "ut", "ux", "uy", "un",
+    "pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn",
+    "Pi",
+)
+NUM_OUTPUTS = len(OUTPUT_NAMES)
+
+FSM_OK = 0
+FSM_ERR_SIZE = -1
+FSM_ERR_IC = -2
+FSM_ERR_FILE = -3
+FSM_ERR_OOM = -4
+FSM_ERR_DEVICE = -5
+FSM_ERR_ARG = -6
+FSM_ERR_UNSUPPORTED = -7
+FSM_ERR_BUSY = -8
+FSM_ERR_CANCELLED = -9
+
+PATH_FP32 = 0
+PATH_MIXED = 1
+
+_PARAM_FIELDS = [
+    ("OUTPUTFORMAT", ctypes.c_int), ("BARYON", ctypes.c_int), ("IC_ENERGY", ctypes.c_int),
+    ("IC_BARYON", ctypes.c_int), ("ETA_WIDTH", ctypes.c_float), ("ETA_FLAT", ctypes.c_float),
+    ("SIGMA", ctypes.c_float), ("SIGMA_B", ctypes.c_float), ("DIM_X", ctypes.c_int),
+    ("DIM_Y", ctypes.c_int), ("DIM_ETA", ctypes.c_int), ("DIM_RAP", ctypes.c_int),
+    ("DIM_PHIP", ctypes.c_int), ("DX", ctypes.c_float), ("DY", ctypes.c_float),
+    ("DETA", ctypes.c_float), ("DRAP", ctypes.c_float), ("DTAU", ctypes.c_float),
+    ("TAU0", ctypes.c_float), ("EOS_TYPE", ctypes.c_int), ("E_FREEZE", ctypes.c_float),
+    ("VISCOUS_MATCHING", ctypes.c_int), ("E_DEP_FS", ctypes.c_int), ("E_R", ctypes.c_float),
+    ("TAU_R", ctypes.c_float), ("ALPHA", ctypes.c_float), ("DIM", ctypes.c_int),
+    ("TAU", ctypes.c_float),
+]
+
+
+class Params(ctypes.Structure):
+    """struct parameters of src/Parameter.h:2-33 (same fields, same order)."""
+
+    _fields_ = _PARAM_FIELDS
+
+    def copy(self) -> "Params":
+        q = Params()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(self), ctypes.sizeof(Params))
+        return q
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in _PARAM_FIELDS}
+
+    def update(self, **kw) -> "Params":
+        for k, v in kw.items():
+            if k not in dict(_PARAM_FIELDS):
+                raise KeyError(k)
+            setattr(self, k, v)
+        return self
+
+    def __repr__(self) -> str:  # pragma: no cover
+        return "Params(%s)" % ", ".join("%s=%r" % kv for kv in self.as_dict().items())
+
+
+class FsmError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__("%s: %s (status %d)" % (what, _lib().fsm_status_string(status).decode(), status))
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libfsm.so not found at %s; build it with `make -C freestream-milne_amd` "
+                "(or __graft_entry__.build())" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(Params)
+        vp = ctypes.c_void_p
+        fp = ctypes.POINTER(ctypes.c_float)
+        lib.fsm_version.restype = ctypes.c_char_p
+        lib.fsm_status_string.restype = ctypes.c_char_p
+        lib.fsm_status_string.argtypes = [ctypes.c_int]
+        lib.fsm_params_default.argtypes = [P]
+        lib.fsm_params_default.restype = None
+        lib.fsm_params_read.argtypes = [P, ctypes.c_char_p]
+        lib.fsm_params_finalize.argtypes = [P, fp, ctypes.c_size_t]
+        lib.fsm_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        lib.fsm_destroy.argtypes = [vp]
+        lib.fsm_destroy.restype = None
+        lib.fsm_set_path.argtypes = [vp, ctypes.c_int]
+        lib.fsm_run.argtypes = [vp, P, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), fp]
+        lib.fsm_run_baryon.argtypes = [vp, P, fp, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), fp]
+        lib.fsm_run_device.argtypes = [vp, P, vp, vp, vp]
+        lib.fsm_stream_tmunu_device.argtypes = [vp, P, vp, vp, vp]
+        lib.fsm_landau_device.argtypes = [vp, P, vp, vp, vp, vp, vp, vp, vp]
+        lib.fsm_sync.argtypes = [vp, vp]
+        lib.fsm_enable_timing.argtypes = [vp, ctypes.c_int]
+        lib.fsm_last_kernel_ms.argtypes = [vp, ctypes.c_int, fp]
+        lib.fsm_kernel_time_totals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+        _LIB = lib
+    return _LIB
+
+
+def lib():
+    """The loaded ctypes library (raises ImportError when not built)."""
+    return _lib()
+
+
+def version() -> str:
+    return _lib().fsm_version().decode()
+
+
+def _check(st: int, what: str):
+    if st != FSM_OK:
+        raise FsmError(st, what)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def params_default() -> Params:
+    """Built-in defaults, src/FreestreamMilne.cpp:149-171."""
+    p = Params()
+    _lib().fsm_params_default(ctypes.byref(p))
+    return p
+
+
+def params_read(path: str, p: Params | None = None) -> tuple[Params, int]:
+    """Positional freestream_input reader (src/FileIO.cpp:166-245).
+
+    Returns (params, status); a missing file keeps the given/default params
+    and returns FSM_ERR_FILE, as the reference keeps its defaults."""
+    p = params_default() if p is None else p.copy()
+    st = _lib().fsm_params_read(ctypes.byref(p), path.encode())
+    return p, st
+
+
+def params_finalize(p: Params, e: np.ndarray | None = None) -> Params:
+    """Derived fields and run-time overrides (src/FreestreamMilne.cpp:177-178,316-332)."""
+    q = p.copy()
+    if e is None:
+        st = _lib().fsm_params_finalize(ctypes.byref(q), None, 0)
+    else:
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        st = _lib().fsm_params_finalize(ctypes.byref(q), _fptr(e), e.size)
+    _check(st, "fsm_params_finalize")
+    return q
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    h = getattr(stream, "cuda_stream", None)
+    if h is not None:
+        return h
+    raise TypeError("stream must be None, an int handle or a torch.cuda.Stream")
+
+
+class Context:
+    """One device context (fsm_create/fsm_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = ctypes.c_void_p()
+        _check(_lib().fsm_create(device, ctypes.byref(self._ctx)), "fsm_create")
+        self.device = device
+
+    def close(self):
+        if self._ctx:
+            _lib().fsm_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_path(self, path: int):
+        _check(_lib().fsm_set_path(self._ctx, path), "fsm_set_path")
+
+    # -- host-buffer step ------------------------------------------------
+    def run(self, p: Params, e: np.ndarray):
+        """Whole step on host buffers.  Returns (outputs dict, finalized params, tau)."""
+        e = np.ascontiguousarray(e, dtype=np.float32).ravel()
+        q = p.copy()
+        n = e.size
+        outs = [np.empty(n, dtype=np.float64) for _ in range(NUM_OUTPUTS)]
+        arr = (ctypes.c_void_p * NUM_OUTPUTS)(*[o.ctypes.data for o in outs])
+        tau = ctypes.c_float()
+        st = _lib().fsm_run(self._ctx, ctypes.byref(q), _fptr(e), n, arr, ctypes.byref(tau))
+        _check(st, "fsm_run")
+        return dict(zip(OUTPUT_NAMES, outs)), q, tau.value
+
+    def run_baryon(self, p: Params, e: np.ndarray, nB: np.ndarray):
+        e = np.ascontiguousarray(e, dtype=np.float32).ravel()
+        nB = np.ascontiguousarray(nB, dtype=np.float32).ravel()
+        q = p.copy()
+        n = e.size
+        outs = [np.empty(n, dtype=np.float64) for _ in range(NUM_OUTPUTS)]
+        nb_out = np.empty(n, dtype=np.float64)
+        V = [np.empty(n, dtype=np.float64) for _ in range(4)]
+        arr = (ctypes.c_void_p * NUM_OUTPUTS)(*[o.ctypes.data for o in outs])
+        varr = (ctypes.c_void_p * 4)(*[v.ctypes.data for v in V])
+        tau = ctypes.c_float()
+        st = _lib().fsm_run_baryon(self._ctx, ctypes.byref(q), _fptr(e), _fptr(nB), n, arr,
+                                   nb_out.ctypes.data, varr, ctypes.byref(tau))
+        _check(st, "fsm_run_baryon")
+        return dict(zip(OUTPUT_NAMES, outs)), nb_out, V, q, tau.value
+
+    # -- device-resident entry points (raw device pointers) --------------
+    def run_device(self, p: Params, d_e: int, d_out: int, stream=None):
+        _check(_lib().fsm_run_device(self._ctx, ctypes.byref(p), d_e, d_out, _stream_handle(stream)),
+               "fsm_run_device")
+
+    def stream_tmunu_device(self, p: Params, d_e: int, d_T: int, stream=None):
+        _check(_lib().fsm_stream_tmunu_device(self._ctx, ctypes.byref(p), d_e, d_T, _stream_handle(stream)),
+               "fsm_stream_tmunu_device")
+
+    def landau_device(self, p: Params, d_T: int, d_e=None, d_u=None, d_p=None, d_Pi=None, d_pi=None,
+                      stream=None):
+        _check(_lib().fsm_landau_device(self._ctx, ctypes.byref(p), d_T, d_e, d_u, d_p, d_Pi, d_pi,
+                                        _stream_handle(stream)), "fsm_landau_device")
+
+    def sync(self, stream=None):
+        _check(_lib().fsm_sync(self._ctx, _stream_handle(stream)), "fsm_sync")
+
+    def enable_timing(self, on: bool = True):
+        _check(_lib().fsm_enable_timing(self._ctx, 1 if on else 0), "fsm_enable_timing")
+
+    def kernel_time_totals(self):
+        """(ms[3] summed over calls since the last collection, number of calls)."""
+        ms = (ctypes.c_double * 3)()
+        n = ctypes.c_int()
+        _check(_lib().fsm_kernel_time_totals(self._ctx, ms, ctypes.byref(n)), "fsm_kernel_time_totals")
+        return list(ms), n.value
+
+    def last_kernel_ms(self, which: int) -> float:
+        v = ctypes.c_float()
+        _check(_lib().fsm_last_kernel_ms(self._ctx, which, ctypes.byref(v)), "fsm_last_kernel_ms")
+        return v.value
+
+
+class FreestreamMilne:
+    """Mirror of class FREESTREAMMILNE (src/FreestreamMilne.cpp:28-87).
+
+    ``run_freestream_milne`` reads ``./freestream_input`` (or ``input_file``)
+    positionally over the built-in defaults, exactly as the reference does, then
+    runs the step on the GPU for IC_ENERGY = 5 (vector input).  A vector whose
+    size differs from DIM raises ``FsmError(FSM_ERR_SIZE)`` where the reference
+    calls exit(-1) (src/FreestreamMilne.cpp:259-264)."""
+
+    def __init__(self, device: int = 0, input_file: str = "freestream_input", context: Context | None = None):
+        self._ctx = context if context is not None else Context(device)
+        self.input_file = input_file
+        self.gridSize = 0
+        self.tau_LandauMatch = 0.0
+        self.init_energy_density = np.zeros(0, dtype=np.float32)
+        self.params = None
+        for name in _FINAL_NAMES:
+            setattr(self, name, np.zeros(0, dtype=np.float64))
+
+    def initialize_from_vector(self, energy_density_in):
+        self.init_energy_density = np.array(energy_density_in, dtype=np.float32).ravel()
+
+    def run_freestream_milne(self, params: Params | None = None) -> int:
+        if params is None:
+            params, _ = params_read(self.input_file)
+        if params.IC_ENERGY != 5:
+            raise FsmError(FSM_ERR_IC, "run_freestream_milne (only IC_ENERGY=5 vector input)")
+        outs, q, tau = self._ctx.run(params, self.init_energy_density)
+        self.params = q
+        self.tau_LandauMatch = tau
+        for name, key in zip(_FINAL_NAMES, OUTPUT_NAMES):
+            setattr(self, name, outs[key])
+        return 0
+
+    def output_to_vectors(self):
+        return tuple(getattr(self, n).copy() for n in _FINAL_NAMES)
+
+
+_FINAL_NAMES = (
+    "final_energy_density", "final_pressure", "final_ut", "final_ux", "final_uy", "final_un",
+    "final_pitt", "final_pitx", "final_pity", "final_pitn", "final_pixx", "final_pixy", "final_pixn",
+    "final_piyy", "final_piyn", "final_pinn", "final_Pi",
+)
+
+
+def exported_symbols_from_header(header_path: str) -> list[str]:
+    """Function names declared in include/fsm.h (used by the ABI test)."""
+    import re
+
+    txt = open(header_path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fsm_[a-z0-9_]+)\s*\(", txt)))
+
+
+__all__ = [
+    "Params", "Context", "FreestreamMilne", "FsmError", "OUTPUT_NAMES", "NUM_OUTPUTS", "params_default",
+    "params_read", "params_finalize", "lib", "version", "LIB_PATH", "PATH_FP32", "PATH_MIXED",
+]

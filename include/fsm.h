@@ -1,0 +1,210 @@
+/*
+ * fsm.h -- C ABI of the MI355X free-streaming + Landau-matching library
+ * (libfsm.so, built from freestream-milne_amd/csrc).
+ *
+ * The reference exposes this path as a source-level C++ class
+ * (class FREESTREAMMILNE, src/FreestreamMilne.cpp:28-87) whose per-call
+ * configuration is the positional file ./freestream_input
+ * (src/FileIO.cpp:166-245) and whose results are 17 std::vector<double>
+ * (src/FreestreamMilne.cpp:102-136, filled at :677-718).  This header is the
+ * plain-pointer boundary that replaces it: every entry point takes POD
+ * structs, raw pointers and sizes, and returns an int status.  The header-only
+ * C++ class in freestream-milne_amd/csrc/FreestreamMilne.h rebuilds the
+ * reference class on top of it.
+ *
+ * Units follow the reference: energy density in GeV/fm^3 at the boundary,
+ * fm^-4 inside, lengths in fm, times in fm/c.
+ *
+ * Cell index (every per-cell array): is = (DIM_Y*DIM_ETA)*ix + DIM_ETA*iy + ieta
+ * (src/FreeStream.cpp:159-161).  Stress-tensor component order:
+ * tt, tx, ty, tn, xx, xy, xn, yy, yn, nn (src/LandauMatch.cpp:46-55).
+ */
+#ifndef FSM_H_
+#define FSM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors struct parameters, src/Parameter.h:2-33, field for field and in
+ * the same order, so a reference caller can memcpy its struct. */
+typedef struct fsm_params {
+  int OUTPUTFORMAT;
+  int BARYON;
+  int IC_ENERGY;
+  int IC_BARYON;
+  float ETA_WIDTH;
+  float ETA_FLAT;
+  float SIGMA;
+  float SIGMA_B;
+  int DIM_X;
+  int DIM_Y;
+  int DIM_ETA;
+  int DIM_RAP;
+  int DIM_PHIP;
+  float DX;
+  float DY;
+  float DETA;
+  float DRAP;
+  float DTAU;
+  float TAU0;
+  int EOS_TYPE;
+  float E_FREEZE;
+  int VISCOUS_MATCHING;
+  int E_DEP_FS;
+  float E_R;
+  float TAU_R;
+  float ALPHA;
+  /* derived: DIM = DIM_X*DIM_Y*DIM_ETA, TAU = TAU0 + DTAU */
+  int DIM;
+  float TAU;
+} fsm_params;
+
+/* Status codes.  The reference's conventions are exit(-1) on a vector-size
+ * mismatch (src/FreestreamMilne.cpp:259-264), exit(1) on a missing profile
+ * file (src/FileIO.cpp:150-154) and return 0 with no outputs on an invalid
+ * initial condition (src/FreestreamMilne.cpp:277-281); the library returns a
+ * code instead and the C++ shim maps it back. */
+enum {
+  FSM_OK = 0,
+  FSM_ERR_SIZE = -1,        /* input vector size != DIM */
+  FSM_ERR_IC = -2,          /* IC_ENERGY / IC_BARYON not supported on this path */
+  FSM_ERR_FILE = -3,        /* parameter or profile file missing */
+  FSM_ERR_OOM = -4,         /* device or host allocation failed */
+  FSM_ERR_DEVICE = -5,      /* a HIP call failed */
+  FSM_ERR_ARG = -6,         /* invalid argument (NULL pointer, bad size) */
+  FSM_ERR_UNSUPPORTED = -7, /* parameter combination outside this library */
+  FSM_ERR_BUSY = -8,        /* serving queue full / no slot free */
+  FSM_ERR_CANCELLED = -9    /* request cancelled before it ran */
+};
+
+/* Number and order of the OUTPUTFORMAT=2 result vectors:
+ * e, p, ut, ux, uy, un, pitt, pitx, pity, pitn, pixx, pixy, pixn, piyy,
+ * piyn, pinn, Pi  (src/FreestreamMilne.cpp:51-67, :700-716). */
+#define FSM_NUM_OUTPUTS 17
+
+/* Library version string. */
+const char *fsm_version(void);
+const char *fsm_status_string(int status);
+
+/* Built-in defaults: src/FreestreamMilne.cpp:149-171 (E_R, TAU_R, ALPHA have
+ * no reference default; they are set to 0 here).  Also fills DIM and TAU. */
+void fsm_params_default(fsm_params *p);
+
+/* Positional reader of a freestream_input file: src/FileIO.cpp:166-245.
+ * Reads 26 "NAME value" lines in the fixed reference order, ignoring the
+ * names.  Returns FSM_ERR_FILE (and leaves *p untouched) if the file cannot
+ * be opened -- the reference prints a message and keeps its defaults. */
+int fsm_params_read(fsm_params *p, const char *path);
+
+/* Derived fields and run-time overrides, in the reference order:
+ *   DIM, TAU                       src/FreestreamMilne.cpp:177-178
+ *   E_DEP_FS: DTAU = TAU_R*(e_T/E_R)^ALPHA from the converted input
+ *                                  src/FreestreamMilne.cpp:316-323,
+ *                                  src/FreeStream.cpp:389-417
+ *   DIM_PHIP = max(DIM_PHIP, ceil(2 pi DTAU / min(DX,DY)))
+ *                                  src/FreestreamMilne.cpp:329-332
+ * e_GeV (length n, may be NULL when E_DEP_FS == 0) is the input vector in
+ * GeV/fm^3.  Returns FSM_ERR_SIZE if e_GeV != NULL and n != DIM. */
+int fsm_params_finalize(fsm_params *p, const float *e_GeV, size_t n);
+
+/* ------------------------------------------------------------------ */
+/* Context: one per host thread; owns device workspace and event tables. */
+typedef struct fsm_ctx fsm_ctx;
+
+/* Creates a context on HIP device `device`. */
+int fsm_create(int device, fsm_ctx **out);
+void fsm_destroy(fsm_ctx *ctx);
+
+/* Quadrature path selector for the streaming stage. */
+enum {
+  FSM_PATH_FP32 = 0,  /* FP32 VALU interpolation and quadrature */
+  FSM_PATH_MIXED = 1  /* FP32 interpolation, quadrature on FP16 MFMA with
+                         hi+lo split of samples and weights */
+};
+int fsm_set_path(fsm_ctx *ctx, int path);
+
+/* ------------------------------------------------------------------ */
+/* Whole step with host buffers (the drop-in call).
+ *
+ * Replaces FREESTREAMMILNE::run_freestream_milne() with IC_ENERGY=5
+ * (src/FreestreamMilne.cpp:139-741, vector ingest :251-271) followed by
+ * output_to_vectors() (src/FreestreamMilne.cpp:102-136).
+ *   p        in: parameters as read (defaults + file); out: finalized
+ *            (DIM, TAU, DTAU and DIM_PHIP overrides applied).
+ *   e_GeV    initial energy density, n floats, GeV/fm^3.
+ *   out      FSM_NUM_OUTPUTS caller-owned arrays of DIM doubles each, in
+ *            the order documented above.  When OUTPUTFORMAT is not 2 or 3
+ *            they are zero-filled, as the reference leaves them
+ *            (src/FreestreamMilne.cpp:695).
+ *   tau_landau_match  out (may be NULL): TAU0 + DTAU after overrides
+ *            (src/FreestreamMilne.cpp:326).
+ * Synchronous; the library keeps no pointer to caller memory. */
+int fsm_run(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, size_t n,
+            double *const out[FSM_NUM_OUTPUTS], float *tau_landau_match);
+
+/* Same, plus the baryon extension (BARYON=1): nB_in is the initial baryon
+ * density (n floats, fm^-3) fed through convertInitialChargeDensity
+ * (src/FreeStream.cpp:353-387), a second freeStream pass
+ * (src/FreestreamMilne.cpp:386) and calculateBaryonCurrent /
+ * calculateBaryonDensity / calculateBaryonDiffusion
+ * (src/LandauMatch.cpp:110-143, :370-392).  The reference writes these
+ * results to files only (src/FreestreamMilne.cpp:666-673); here they come
+ * back as nB_out (DIM doubles) and V_out[4] (DIM doubles each, V^tau, V^x,
+ * V^y, V^eta).  Either output may be NULL. */
+int fsm_run_baryon(fsm_ctx *ctx, fsm_params *p, const float *e_GeV,
+                   const float *nB_in, size_t n,
+                   double *const out[FSM_NUM_OUTPUTS], double *nB_out,
+                   double *const V_out[4], float *tau_landau_match);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident entry points.  All pointers are device pointers; `stream`
+ * is a hipStream_t (NULL = default stream).  `p` must be finalized.  Calls
+ * are asynchronous on `stream`; tables for the event shape are built on the
+ * first call with that shape and cached in the context. */
+
+/* Whole step: e_GeV[DIM] -> out[17*DIM] doubles (SoA, FSM_NUM_OUTPUTS planes
+ * of DIM in the output order).  Equivalent to fsm_run without host copies. */
+int fsm_run_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV,
+                   double *d_out, void *stream);
+
+/* Stage A: convertInitialDensity + freeStream + calculateHypertrigTable +
+ * calculateStressTensor fused (src/FreeStream.cpp:97-350,
+ * src/LandauMatch.cpp:14-108).  Input e_GeV[DIM] in GeV/fm^3 (the ingest of
+ * src/FreestreamMilne.cpp:267 is applied inside); output d_T[10*DIM]
+ * floats, 10 planes of DIM in the reference layout (stressTensor[10][DIM]). */
+int fsm_stream_tmunu_device(fsm_ctx *ctx, const fsm_params *p,
+                            const float *d_e_GeV, float *d_T, void *stream);
+
+/* Stage B: solveEigenSystem + calculatePressure + calculateBulkPressure +
+ * calculateShearViscTensor (src/LandauMatch.cpp:145-367,
+ * src/EquationOfState.cpp:5-80) on d_T[10*DIM] (reference layout, fm^-4).
+ * Float outputs in fm^-4 (any may be NULL): d_e[DIM], d_u[4*DIM],
+ * d_p[DIM], d_Pi[DIM], d_pi[10*DIM]. */
+int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T,
+                      float *d_e, float *d_u, float *d_p, float *d_Pi,
+                      float *d_pi, void *stream);
+
+/* Waits for all work the context queued on `stream`. */
+int fsm_sync(fsm_ctx *ctx, void *stream);
+
+/* Kernel-time counters of the last device call on this context (HIP events
+ * recorded around each kernel on the launch stream), in milliseconds.
+ * which: 0 = spline precompute, 1 = stream + quadrature, 2 = Landau + output
+ * pack.  Timing is recorded only after fsm_enable_timing(ctx, 1). */
+int fsm_enable_timing(fsm_ctx *ctx, int on);
+int fsm_last_kernel_ms(fsm_ctx *ctx, int which, float *ms);
+
+/* Sums of the same per-kernel times over every device call made since timing
+ * was enabled or last collected (ms[3], in milliseconds), and the number of
+ * calls; resets the sums.  Waits for the recorded work to finish. */
+int fsm_kernel_time_totals(fsm_ctx *ctx, double ms[3], int *ncalls);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSM_H_ */

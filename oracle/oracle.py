@@ -1,0 +1,167 @@
+"""ctypes loader of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.  The oracle is a
+C++ restatement of the reference algorithm (see fsm_oracle.cpp's header for
+the reference files it follows and for how each part is pinned).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+_LIB = None
+
+
+def build(force: bool = False):
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-C", _HERE, "-s"])
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        build()
+        lib = ctypes.CDLL(LIB_PATH)
+        from fsm_amd import Params  # same struct layout as include/fsm.h
+
+        P = ctypes.POINTER(Params)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        lib.oracle_params_default.argtypes = [P]
+        lib.oracle_params_default.restype = None
+        lib.oracle_params_finalize.argtypes = [P, fp, ctypes.c_long]
+        lib.oracle_run.argtypes = [P, fp, fp, ctypes.c_int, dp, dp, dp, dp]
+        lib.oracle_spline_derivs.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp]
+        lib.oracle_spline_derivs.restype = None
+        lib.oracle_spline_eval.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_int, dp, dp, dp]
+        lib.oracle_spline_eval.restype = None
+        lib.oracle_eig4.argtypes = [dp, dp, dp, dp]
+        lib.oracle_landau.argtypes = [ctypes.c_long, dp, ctypes.c_float, dp, dp]
+        lib.oracle_landau.restype = None
+        lib.oracle_num_threads.restype = ctypes.c_int
+        _LIB = lib
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def params_default():
+    from fsm_amd import Params
+
+    p = Params()
+    _lib().oracle_params_default(ctypes.byref(p))
+    return p
+
+
+def params_finalize(p, e=None):
+    q = p.copy()
+    if e is None:
+        st = _lib().oracle_params_finalize(ctypes.byref(q), None, 0)
+    else:
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        st = _lib().oracle_params_finalize(ctypes.byref(q), _f(e), e.size)
+    if st != 0:
+        raise ValueError("oracle_params_finalize status %d" % st)
+    return q
+
+
+def run(p, e, precision: str = "float", nB=None, want_T: bool = True, want_extra: bool = True):
+    """Whole step.  precision 'float' = reference float path, 'double' = FP64 oracle.
+
+    p must be finalized.  Returns dict with 'out' (17 x DIM, output_to_vectors
+    order), 'T' (10 x DIM, fm^-4), 'e', 'u' (fm^-4 / dimensionless), optional
+    'nB', 'V' (baryon), and 'stage_seconds'."""
+    e = np.ascontiguousarray(e, dtype=np.float32).ravel()
+    D = p.DIM
+    if e.size != D:
+        raise ValueError("size mismatch")
+    out = np.empty((17, D))
+    T = np.empty((10, D)) if want_T else None
+    extra = np.zeros((10, D)) if want_extra else None
+    secs = np.zeros(4)
+    nb = None if nB is None else np.ascontiguousarray(nB, dtype=np.float32).ravel()
+    st = _lib().oracle_run(ctypes.byref(p), _f(e), None if nb is None else _f(nb),
+                           0 if precision == "float" else 1, _d(out),
+                           None if T is None else _d(T), None if extra is None else _d(extra), _d(secs))
+    if st != 0:
+        raise ValueError("oracle_run status %d" % st)
+    r = {"out": out, "T": T, "stage_seconds": secs}
+    if extra is not None:
+        r["e"] = extra[0]
+        r["u"] = extra[1:5]
+        if nb is not None:
+            r["nB"] = extra[5]
+            r["V"] = extra[6:10]
+    return r
+
+
+def spline_derivs(x, y, z):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    nx, ny = x.size, y.size
+    zx, zy, zxy = (np.empty((nx, ny)) for _ in range(3))
+    _lib().oracle_spline_derivs(nx, ny, _d(x), _d(y), _d(z), _d(zx), _d(zy), _d(zxy))
+    return zx, zy, zxy
+
+
+def spline_eval(x, y, z, px, py):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    out = np.empty(px.size)
+    _lib().oracle_spline_eval(x.size, y.size, _d(x), _d(y), _d(z), px.size, _d(px), _d(py), _d(out))
+    return out
+
+
+def eig4(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    wr, wi, V = np.empty(4), np.empty(4), np.empty((4, 4))
+    ok = _lib().oracle_eig4(_d(A), _d(wr), _d(wi), _d(V))
+    return wr, wi, V, ok
+
+
+def landau(T, tau: float):
+    """T: (n, 10) doubles -> e (n,), u (n, 4) by the reference's selection rules."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    n = T.shape[0]
+    e, u = np.empty(n), np.empty((n, 4))
+    _lib().oracle_landau(n, _d(T), tau, _d(e), _d(u))
+    return e, u
+
+
+def num_threads() -> int:
+    return _lib().oracle_num_threads()
+
+
+def landau_condition(T, tau: float, ntrial: int = 4, rel: float = 1e-6, seed: int = 12345):
+    """Per-cell condition numbers (kappa_e, kappa_u) of the Landau matching:
+    max relative change of e (and of u^mu in units of u^tau) per unit relative
+    perturbation of T, over `ntrial` random perturbations.  T: (n, 10)."""
+    lib = _lib()
+    if not hasattr(lib, "_cond_ready"):
+        lib.oracle_landau_condition.argtypes = [ctypes.c_long, ctypes.POINTER(ctypes.c_double), ctypes.c_float,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_uint,
+                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        lib.oracle_landau_condition.restype = None
+        lib._cond_ready = True
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    n = T.shape[0]
+    ke, ku = np.empty(n), np.empty(n)
+    lib.oracle_landau_condition(n, _d(T), tau, ntrial, rel, seed, _d(ke), _d(ku))
+    return ke, ku
