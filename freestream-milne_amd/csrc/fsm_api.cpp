@@ -11,9 +11,12 @@
 #include <array>
 #include <cmath>
 #include <memory>
+#include <mutex>
+#include <new>
 #include <vector>
 
 #include "../../include/fsm.h"
+#include "fsm_host.h"
 #include "fsm_internal.h"
 #include "fsm_kernels.h"
 
@@ -138,13 +141,15 @@ struct fsm_ctx {
   int path = FSM_PATH_FP32;
   int timing = 0;
   int max_lds = 65536;
+  int num_cus = 256;
   // Timing: one set of 4 events per device call (stage boundaries), pooled
   // so that a run of calls can be summed without synchronising in between.
   std::vector<std::array<hipEvent_t, 4>> pool;
   int used = 0;   // sets recorded since the last collection
   int last = -1;  // set of the most recent call
   float last_ms[3] = {0, 0, 0};
-  std::unique_ptr<Plan> plan;
+  std::vector<std::shared_ptr<Plan>> plans;
+  std::mutex plan_mu;  // the plan cache is shared with the server's worker thread
   // host-API workspace
   DevBuf<float> d_in, d_T;
   DevBuf<double> d_out;
@@ -231,13 +236,14 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
       sg.bx1 = oxmax + 1;
       sg.by0 = oymin;
       sg.by1 = oymax + 1;
-      int need = (FSM2D_TX + sg.bx1 - sg.bx0) * (FSM2D_TY + sg.by1 - sg.by0);
+      // box of the widest tile shape (variant 0)
+      int need = (fsm_stream2d_tile_w(0) + sg.bx1 - sg.bx0) * (fsm_stream2d_tile_h(0) + sg.by1 - sg.by0);
       lds_max = std::max(lds_max, need);
       groups.push_back(sg);
     }
     if (lds_max <= budget4 || G >= nphi) break;
   }
-  if ((size_t)lds_max * sizeof(float4) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
+  if (fsm_stream2d_lds_bytes(1, lds_max) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
   FSM_TRY(pl.gx.upload(gxw));
   FSM_TRY(pl.gy.upload(gyw));
   FSM_TRY(pl.st.upload(st));
@@ -252,7 +258,19 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.out_scale = (float)p.DIM_RAP;
   ev.d_phip = (float)((2.0 * M_PI) / (double)(float)nphi);
   ev.tau = p.TAU;
-  ev.tiles_x = (nx + FSM2D_TX - 1) / FSM2D_TX;
+  // One-event launches pick the widest tile shape that still gives at least
+  // two workgroups per CU (the batch path always uses variant 0).
+  ev.variant = 2;
+  for (int v = 0; v < 3; ++v) {
+    int tw = fsm_stream2d_tile_w(v), th = fsm_stream2d_tile_h(v);
+    long tiles = (long)((nx + tw - 1) / tw) * ((ny + th - 1) / th);
+    if (tiles >= 2L * ctx->num_cus) {
+      ev.variant = v;
+      break;
+    }
+  }
+  ev.tiles_x = (nx + fsm_stream2d_tile_w(ev.variant) - 1) / fsm_stream2d_tile_w(ev.variant);
+  ev.tiles_y = (ny + fsm_stream2d_tile_h(ev.variant) - 1) / fsm_stream2d_tile_h(ev.variant);
   ev.xmin = (float)((-1.0) * ((double)(float)(nx - 1) / 2.0) * (double)p.DX);
   ev.ymin = (float)((-1.0) * ((double)(float)(ny - 1) / 2.0) * (double)p.DY);
   ev.dx = p.DX;
@@ -363,21 +381,39 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   return FSM_OK;
 }
 
-int get_plan(fsm_ctx *ctx, const fsm_params &p, Plan **out) {
-  if (ctx->plan && ctx->plan->valid && same_shape(ctx->plan->key, p)) {
-    *out = ctx->plan.get();
-    return FSM_OK;
+// Plans are cached per parameter set (most recently used first).  A batch
+// keeps shared references to the plans of its events, so eviction from the
+// cache never frees tables a recorded batch still uses.
+int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *out) {
+  std::lock_guard<std::mutex> lock(ctx->plan_mu);
+  for (size_t i = 0; i < ctx->plans.size(); ++i) {
+    if (same_shape(ctx->plans[i]->key, p)) {
+      std::shared_ptr<Plan> hit = ctx->plans[i];
+      ctx->plans.erase(ctx->plans.begin() + i);
+      ctx->plans.insert(ctx->plans.begin(), hit);
+      *out = hit;
+      return FSM_OK;
+    }
   }
   int st = check_supported(p);
   if (st) return st;
-  std::unique_ptr<Plan> pl(new Plan());
+  std::shared_ptr<Plan> pl(new Plan());
   pl->key = p;
   pl->is3d = p.DIM_ETA > 1;
   st = pl->is3d ? build_plan3d(ctx, p, *pl) : build_plan2d(ctx, p, *pl);
   if (st) return st;
   pl->valid = true;
-  ctx->plan = std::move(pl);
-  *out = ctx->plan.get();
+  ctx->plans.insert(ctx->plans.begin(), pl);
+  if (ctx->plans.size() > 64) ctx->plans.pop_back();
+  *out = pl;
+  return FSM_OK;
+}
+
+int get_plan(fsm_ctx *ctx, const fsm_params &p, Plan **out) {
+  std::shared_ptr<Plan> sp;
+  int st = get_plan_shared(ctx, p, &sp);
+  if (st) return st;
+  *out = sp.get();
   return FSM_OK;
 }
 
@@ -505,6 +541,7 @@ int fsm_create(int device, fsm_ctx **out) {
   // 3-D planes are staged whole when they fit; keep two workgroups per CU.
   int lds = (int)std::min<size_t>(prop.sharedMemPerBlock, (size_t)80 * 1024);
   ctx->max_lds = lds;
+  ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   FSM_TRY(fsm_set_lds_limits(lds, lds));
   *out = ctx.release();
   return FSM_OK;
@@ -645,3 +682,308 @@ int fsm_run_baryon(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, const float 
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Batches
+// ============================================================================
+namespace {
+
+struct BatchEvent {
+  std::shared_ptr<Plan> plan;
+  fsm_params p;
+  const float *d_e;
+  double *d_out;
+  char *ws = nullptr;  // caller-provided workspace (event_workspace_bytes), else the batch arena
+};
+
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Device workspace of one event in a batch: spline planes + T (2-D), or
+// source planes + eta-major T (3-D).
+size_t event_workspace_bytes(const fsm_params &p) {
+  if (p.DIM_ETA == 1)
+    return align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4)) + align256((size_t)10 * p.DIM * sizeof(float));
+  return align256((size_t)p.DIM * sizeof(float)) + align256((size_t)10 * p.DIM * sizeof(float));
+}
+
+// Host staging for stream-ordered uploads (pinned, grown as needed).
+struct PinnedBuf {
+  char *p = nullptr;
+  size_t n = 0;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipHostMalloc((void **)&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+    if (e == hipSuccess) n = std::max<size_t>(bytes, 256);
+    return e;
+  }
+};
+
+}  // namespace
+
+struct fsm_batch {
+  fsm_ctx *ctx = nullptr;
+  std::vector<BatchEvent> evs;
+  bool dirty = true;
+  int use_graph = 1;
+  DevBuf<char> arena;
+  DevBuf<Event2D> d_ev2;
+  DevBuf<Event3D> d_ev3;
+  DevBuf<EventLandau> d_evl;
+  DevBuf<int4> d_items;
+  int off_sp = 0, n_sp = 0, off_s2 = 0, n_s2 = 0, off_src = 0, n_src = 0, off_s3 = 0, n_s3 = 0, off_l = 0,
+      n_l = 0;
+  size_t lds2 = 0, lds3 = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // stream-ordered uploads (used by the server): descriptors go through
+  // pinned staging with hipMemcpyAsync on the launch stream
+  bool async_upload = false;
+  PinnedBuf staging;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~fsm_batch() { drop_graph(); }
+};
+
+namespace {
+
+const int kLandauChunk = 1024;
+
+int batch_build(fsm_batch *b, hipStream_t s) {
+  b->drop_graph();
+  std::vector<Event2D> e2;
+  std::vector<Event3D> e3;
+  std::vector<EventLandau> el;
+  std::vector<int4> sp, s2, src, s3, ld;
+  size_t bytes = 0;
+  for (auto &ev : b->evs)
+    if (!ev.ws) bytes += event_workspace_bytes(ev.p);
+  if (bytes) FSM_TRY(b->arena.ensure(bytes));
+  char *arena_cur = b->arena.p;
+  b->lds2 = b->lds3 = 0;
+  for (auto &ev : b->evs) {
+    const fsm_params &p = ev.p;
+    const int li = (int)el.size();
+    char *cur = ev.ws;
+    if (!cur) {
+      cur = arena_cur;
+      arena_cur += event_workspace_bytes(p);
+    }
+    if (!ev.plan->is3d) {
+      Event2D d = ev.plan->ev2;
+      d.e_gev = ev.d_e;
+      d.planes = (float4 *)cur;
+      cur += align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4));
+      d.T = (float *)cur;
+      cur += align256((size_t)10 * p.DIM * sizeof(float));
+      d.t_plane = p.DIM;
+      const int i = (int)e2.size();
+      e2.push_back(d);
+      const int stx = (p.DIM_X + FSMSP_TX - 1) / FSMSP_TX, sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
+      for (int x = 0; x < stx; ++x)
+        for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
+      // the batch stream kernel is variant 0 for every event
+      d.variant = 0;
+      d.tiles_x = (p.DIM_X + fsm_stream2d_tile_w(0) - 1) / fsm_stream2d_tile_w(0);
+      d.tiles_y = (p.DIM_Y + fsm_stream2d_tile_h(0) - 1) / fsm_stream2d_tile_h(0);
+      e2.back() = d;
+      for (int y = 0; y < d.tiles_y; ++y)
+        for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
+      b->lds2 = std::max(b->lds2, fsm_stream2d_lds_bytes(0, d.lds_floats4));
+      EventLandau L = landau_event(p, d.T, 0);
+      L.out = ev.d_out;
+      el.push_back(L);
+    } else {
+      Event3D d = ev.plan->ev3;
+      d.e_gev = ev.d_e;
+      d.src = (float *)cur;
+      cur += align256((size_t)p.DIM * sizeof(float));
+      d.T = (float *)cur;
+      cur += align256((size_t)10 * p.DIM * sizeof(float));
+      d.t_plane = p.DIM;
+      const int i = (int)e3.size();
+      e3.push_back(d);
+      const int nxy = p.DIM_X * p.DIM_Y;
+      for (int x = 0; x < (nxy + 63) / 64; ++x)
+        for (int y = 0; y < (p.DIM_ETA + 63) / 64; ++y) src.push_back(make_int4(i, x, y, 0));
+      for (int ie = 0; ie < p.DIM_ETA; ++ie)
+        for (int t = 0; t < d.tiles; ++t) s3.push_back(make_int4(i, t, ie, 0));
+      b->lds3 = std::max(b->lds3, (size_t)d.lds_floats * sizeof(float));
+      EventLandau L = landau_event(p, d.T, 1);
+      L.out = ev.d_out;
+      el.push_back(L);
+    }
+    for (int c = 0; c < p.DIM; c += kLandauChunk) ld.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
+  }
+  std::vector<int4> items;
+  auto put = [&](std::vector<int4> &v, int *off, int *n) {
+    *off = (int)items.size();
+    *n = (int)v.size();
+    items.insert(items.end(), v.begin(), v.end());
+  };
+  put(sp, &b->off_sp, &b->n_sp);
+  put(s2, &b->off_s2, &b->n_s2);
+  put(src, &b->off_src, &b->n_src);
+  put(s3, &b->off_s3, &b->n_s3);
+  put(ld, &b->off_l, &b->n_l);
+  if (!b->async_upload) {
+    if (!e2.empty()) FSM_TRY(b->d_ev2.upload(e2));
+    if (!e3.empty()) FSM_TRY(b->d_ev3.upload(e3));
+    FSM_TRY(b->d_evl.upload(el));
+    FSM_TRY(b->d_items.upload(items));
+  } else {
+    // one pinned staging block, four stream-ordered copies
+    const size_t n2 = e2.size() * sizeof(Event2D), n3 = e3.size() * sizeof(Event3D),
+                 nl = el.size() * sizeof(EventLandau), ni = items.size() * sizeof(int4);
+    const size_t o3 = align256(n2), ol = o3 + align256(n3), oi = ol + align256(nl);
+    FSM_TRY(b->staging.ensure(oi + ni));
+    memcpy(b->staging.p, e2.data(), n2);
+    memcpy(b->staging.p + o3, e3.data(), n3);
+    memcpy(b->staging.p + ol, el.data(), nl);
+    memcpy(b->staging.p + oi, items.data(), ni);
+    FSM_TRY(b->d_ev2.ensure(e2.size()));
+    FSM_TRY(b->d_ev3.ensure(e3.size()));
+    FSM_TRY(b->d_evl.ensure(el.size()));
+    FSM_TRY(b->d_items.ensure(items.size()));
+    if (n2) FSM_TRY(hipMemcpyAsync(b->d_ev2.p, b->staging.p, n2, hipMemcpyHostToDevice, s));
+    if (n3) FSM_TRY(hipMemcpyAsync(b->d_ev3.p, b->staging.p + o3, n3, hipMemcpyHostToDevice, s));
+    FSM_TRY(hipMemcpyAsync(b->d_evl.p, b->staging.p + ol, nl, hipMemcpyHostToDevice, s));
+    FSM_TRY(hipMemcpyAsync(b->d_items.p, b->staging.p + oi, ni, hipMemcpyHostToDevice, s));
+  }
+  b->dirty = false;
+  return FSM_OK;
+}
+
+int batch_launch(fsm_batch *b, hipStream_t s, bool timed) {
+  fsm_ctx *ctx = b->ctx;
+  const int4 *it = b->d_items.p;
+  if (timed) tmark(ctx, 0, s);
+  FSM_TRY(fsm_launch_spline2d_batch(b->d_ev2.p, it + b->off_sp, b->n_sp, s));
+  FSM_TRY(fsm_launch_src3d_batch(b->d_ev3.p, it + b->off_src, b->n_src, s));
+  if (timed) tmark(ctx, 1, s);
+  FSM_TRY(fsm_launch_stream2d_batch(b->d_ev2.p, it + b->off_s2, b->n_s2, b->lds2, s));
+  FSM_TRY(fsm_launch_stream3d_batch(b->d_ev3.p, it + b->off_s3, b->n_s3, b->lds3, s));
+  if (timed) tmark(ctx, 2, s);
+  FSM_TRY(fsm_launch_landau_batch(b->d_evl.p, it + b->off_l, b->n_l, s));
+  if (timed) tmark(ctx, 3, s);
+  return FSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsm_batch_create(fsm_ctx *ctx, fsm_batch **out) {
+  if (!ctx || !out) return FSM_ERR_ARG;
+  fsm_batch *b = new (std::nothrow) fsm_batch();
+  if (!b) return FSM_ERR_OOM;
+  b->ctx = ctx;
+  *out = b;
+  return FSM_OK;
+}
+
+void fsm_batch_destroy(fsm_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  delete b;
+}
+
+int fsm_batch_add(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, double *d_out) {
+  if (!b || !p || !d_e_GeV || !d_out) return FSM_ERR_ARG;
+  FSM_TRY(hipSetDevice(b->ctx->device));
+  BatchEvent ev;
+  int st = get_plan_shared(b->ctx, *p, &ev.plan);
+  if (st) return st;
+  ev.p = *p;
+  ev.d_e = d_e_GeV;
+  ev.d_out = d_out;
+  b->evs.push_back(ev);
+  b->dirty = true;
+  return (int)b->evs.size() - 1;
+}
+
+int fsm_batch_clear(fsm_batch *b) {
+  if (!b) return FSM_ERR_ARG;
+  b->evs.clear();
+  b->dirty = true;
+  b->drop_graph();
+  return FSM_OK;
+}
+
+int fsm_batch_size(const fsm_batch *b) { return b ? (int)b->evs.size() : FSM_ERR_ARG; }
+
+int fsm_batch_set_graph(fsm_batch *b, int use_graph) {
+  if (!b) return FSM_ERR_ARG;
+  b->use_graph = use_graph ? 1 : 0;
+  return FSM_OK;
+}
+
+int fsm_batch_run(fsm_batch *b, void *stream) {
+  if (!b) return FSM_ERR_ARG;
+  if (b->evs.empty()) return FSM_OK;
+  FSM_TRY(hipSetDevice(b->ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  if (b->dirty) {
+    int st = batch_build(b, s);
+    if (st) return st;
+  }
+  if (!b->use_graph || s == nullptr) return batch_launch(b, s, b->ctx->timing != 0);
+  if (!b->exec) {
+    FSM_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = batch_launch(b, s, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    FSM_TRY(e);
+    b->graph = g;
+    FSM_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  FSM_TRY(hipGraphLaunch(b->exec, s));
+  return FSM_OK;
+}
+
+}  // extern "C"
+
+// ---- internal entry points for the server (fsm_host.h) ----------------------
+namespace fsm_host {
+
+size_t workspace_bytes(const fsm_params &p) { return event_workspace_bytes(p); }
+
+int batch_add_ws(fsm_batch *b, const fsm_params *p, const float *d_e, double *d_out, void *ws) {
+  int i = fsm_batch_add(b, p, d_e, d_out);
+  if (i >= 0) b->evs[i].ws = (char *)ws;
+  return i;
+}
+
+void batch_set_async_upload(fsm_batch *b, bool on) { b->async_upload = on; }
+
+int batch_launch_now(fsm_batch *b, hipStream_t s) {
+  if (b->evs.empty()) return FSM_OK;
+  if (b->dirty) {
+    int st = batch_build(b, s);
+    if (st) return st;
+  }
+  return batch_launch(b, s, false);
+}
+
+int ctx_device(const fsm_ctx *ctx) { return ctx->device; }
+
+int plan_check(fsm_ctx *ctx, const fsm_params *p) {
+  std::shared_ptr<Plan> sp;
+  return get_plan_shared(ctx, *p, &sp);
+}
+
+}  // namespace fsm_host

@@ -1,0 +1,28 @@
+// fsm_host.h -- internal host-side entry points shared between fsm_api.cpp
+// (contexts, plans, batches) and fsm_server.cpp (the request loop).
+#ifndef FSM_HOST_H_
+#define FSM_HOST_H_
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "../../include/fsm.h"
+
+namespace fsm_host {
+
+// Device workspace one event needs inside a batch (spline or source planes
+// plus the stress tensor).
+size_t workspace_bytes(const fsm_params &p);
+// fsm_batch_add with caller-owned workspace of workspace_bytes(*p).
+int batch_add_ws(fsm_batch *b, const fsm_params *p, const float *d_e, double *d_out, void *ws);
+// Descriptor uploads through pinned staging, ordered on the launch stream.
+void batch_set_async_upload(fsm_batch *b, bool on);
+// Builds (if changed) and launches the batch on `s` without a graph.
+int batch_launch_now(fsm_batch *b, hipStream_t s);
+int ctx_device(const fsm_ctx *ctx);
+// Builds/looks up the plan for p (returns its status: unsupported shapes fail here).
+int plan_check(fsm_ctx *ctx, const fsm_params *p);
+
+}  // namespace fsm_host
+
+#endif  // FSM_HOST_H_

@@ -67,6 +67,8 @@ struct Event2D {
   float d_phip;         // (float)(2 pi / DIM_PHIP)
   float tau;            // TAU
   int tiles_x;          // tiles along x
+  int tiles_y;          // tiles along y
+  int variant;          // stream kernel shape (fsm_stream2d.hip: 0 = M4/JS1, 1 = M4/JS4, 2 = M2/JS4)
   float xmin, ymin;     // grid origin as getX/getY compute it
   float dx, dy;
   const float *e_gev;   // input, GeV/fm^3

@@ -16,4 +16,20 @@ hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s);
 // Raises the dynamic-LDS ceiling of the stream kernels (bytes per workgroup).
 hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d);
 
+// 2-D stream kernel variants (fsm_stream2d.hip): tile size in cells and the
+// dynamic LDS a launch needs for a knot box of box_floats4 float4 elements.
+int fsm_stream2d_tile_w(int variant);
+int fsm_stream2d_tile_h(int variant);
+size_t fsm_stream2d_lds_bytes(int variant, int box_floats4);
+hipError_t fsm_stream2d_set_lds_limit(int bytes);
+
+
+
+// Batched launchers: `items` are int4 work items in device memory,
+// `evs` device arrays of event descriptors (see fsm_api.cpp, fsm_batch).
+hipError_t fsm_launch_spline2d_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s);
+hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s);
+hipError_t fsm_launch_src3d_batch(const Event3D *evs, const int4 *items, int n, hipStream_t s);
+hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s);
+hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s);
 #endif  // FSM_KERNELS_H_

@@ -126,117 +126,18 @@ __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restric
   }
 }
 
-// ======================================================================
-// 2-D streaming + quadrature
-// ======================================================================
-__device__ void stream2d_tile(const Event2D &ev, int tile_x, int tile_y, float4 *lds) {
-  const int nx = ev.nx, ny = ev.ny;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x0 = tile_x * FSM2D_TX, y0 = tile_y * FSM2D_TY;
-  const int iy = y0 + lane;
-  const int ixb = x0 + wave * FSM2D_M;
-  float acc[FSM2D_M][6];
-#pragma unroll
-  for (int m = 0; m < FSM2D_M; ++m)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) acc[m][c] = 0.0f;
-
-  for (int g = 0; g < ev.ngroups; ++g) {
-    const StencilGroup G = ev.groups[g];
-    const int LX = FSM2D_TX + G.bx1 - G.bx0;
-    const int LY = FSM2D_TY + G.by1 - G.by0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < LX * LY; k += blockDim.x) {
-      int lx = k / LY, ly = k - lx * LY;
-      int gx = x0 + G.bx0 + lx, gy = y0 + G.by0 + ly;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = ev.planes[(long)gx * ny + gy];
-      lds[k] = v;
-    }
-    __syncthreads();
-    for (int j = G.j0; j < G.j1; ++j) {
-      const Stencil2D S = ev.st[j];
-      // LDS index of knot (a, b) = (ix + a + ox, iy + b + oy) relative to the box
-      const int base = (wave * FSM2D_M + S.ox - G.bx0) * LY + (lane + S.oy - G.by0);
-      const bool yok = mask_ok(iy, iy + S.oy, S.dyj, ev.ymin, ev.dy, ny);
-      float Zy[FSM2D_M + 1], ZXy[FSM2D_M + 1], Bl[FSM2D_M + 1];
-#pragma unroll
-      for (int a = 0; a <= FSM2D_M; ++a) {
-        const float4 k0 = lds[base + a * LY];
-        const float4 k1 = lds[base + a * LY + 1];
-        Zy[a] = fmaf(S.hy[0], k0.x, fmaf(S.hy[1], k0.y, fmaf(S.hy[2], k1.x, S.hy[3] * k1.y)));
-        ZXy[a] = fmaf(S.hy[0], k0.z, fmaf(S.hy[1], k0.w, fmaf(S.hy[2], k1.z, S.hy[3] * k1.w)));
-        Bl[a] = fmaf(S.ty, k1.x - k0.x, k0.x);
-      }
-#pragma unroll
-      for (int m = 0; m < FSM2D_M; ++m) {
-        float sp = fmaf(S.hx[0], Zy[m], fmaf(S.hx[1], ZXy[m], fmaf(S.hx[2], Zy[m + 1], S.hx[3] * ZXy[m + 1])));
-        float bl = fmaf(S.tx, Bl[m + 1] - Bl[m], Bl[m]);
-        // src/FreeStream.cpp:245-246: spline value if > 0, else bilinear
-        float v = sp > 0.0f ? sp : bl;
-        // src/FreeStream.cpp:213: floor index in [1, N-2] in x and y
-        bool ok = yok && mask_ok(ixb + m, ixb + m + S.ox, S.dxj, ev.xmin, ev.dx, nx);
-        v = ok ? v : 0.0f;
-        acc[m][0] += v;
-        acc[m][1] = fmaf(v, S.c, acc[m][1]);
-        acc[m][2] = fmaf(v, S.s, acc[m][2]);
-        acc[m][3] = fmaf(v, S.cc, acc[m][3]);
-        acc[m][4] = fmaf(v, S.cs, acc[m][4]);
-        acc[m][5] = fmaf(v, S.ss, acc[m][5]);
-      }
-    }
-  }
-  // src/LandauMatch.cpp:104: T = sum * d_phip / TAU.  2-D hypertrig rows 3, 6,
-  // 8, 9 are tanh(0) multiples and vanish.
-  const long P = ev.t_plane;
-#pragma unroll
-  for (int m = 0; m < FSM2D_M; ++m) {
-    int ix = ixb + m;
-    if (ix < nx && iy < ny) {
-      long is = (long)ix * ny + iy;
-      float r[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) r[c] = ((acc[m][c] * ev.out_scale) * ev.d_phip) / ev.tau;
-      ev.T[0 * P + is] = r[0];
-      ev.T[1 * P + is] = r[1];
-      ev.T[2 * P + is] = r[2];
-      ev.T[3 * P + is] = 0.0f;
-      ev.T[4 * P + is] = r[3];
-      ev.T[5 * P + is] = r[4];
-      ev.T[6 * P + is] = 0.0f;
-      ev.T[7 * P + is] = r[5];
-      ev.T[8 * P + is] = 0.0f;
-      ev.T[9 * P + is] = 0.0f;
-    }
-  }
-}
-
-__global__ __launch_bounds__(FSM2D_THREADS) void k_stream2d_one(Event2D ev) {
-  extern __shared__ float4 lds4[];
-  stream2d_tile(ev, blockIdx.x % ev.tiles_x, blockIdx.x / ev.tiles_x, lds4);
-}
-
-__global__ __launch_bounds__(FSM2D_THREADS) void k_stream2d_batch(const Event2D *__restrict__ evs,
-                                                                  const int4 *__restrict__ items,
-                                                                  int n_items) {
-  extern __shared__ float4 lds4[];
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];
-    const Event2D ev = evs[item.x];
-    stream2d_tile(ev, item.y, item.z, lds4);
-  }
-}
+// The 2-D streaming + quadrature kernels live in fsm_stream2d.hip.
 
 // ======================================================================
 // 3-D source planes and streaming
 // ======================================================================
 // e_gev (reference layout [nxy][neta]) -> src (eta-major [neta][nxy]) with
 // src = e0 * norm_factor (src/FreeStream.cpp:346, first product).
-__global__ __launch_bounds__(256) void k_src3d(const float *__restrict__ e_gev, float *__restrict__ src,
-                                               int nxy, int neta, float norm_factor) {
-  __shared__ float tile[64][65];
-  const int xy0 = blockIdx.x * 64, et0 = blockIdx.y * 64;
+__device__ void src3d_tile(const float *__restrict__ e_gev, float *__restrict__ src, int nxy, int neta,
+                           float norm_factor, int bx, int by, float (*tile)[65]) {
+  const int xy0 = bx * 64, et0 = by * 64;
   const int tx = threadIdx.x & 63, tyb = threadIdx.x >> 6;
+  __syncthreads();
   for (int r = tyb; r < 64; r += 4) {
     int xy = xy0 + r, et = et0 + tx;
     float v = 0.0f;
@@ -247,6 +148,23 @@ __global__ __launch_bounds__(256) void k_src3d(const float *__restrict__ e_gev, 
   for (int r = tyb; r < 64; r += 4) {
     int et = et0 + r, xy = xy0 + tx;
     if (xy < nxy && et < neta) src[(long)et * nxy + xy] = tile[tx][r];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_src3d(const float *__restrict__ e_gev, float *__restrict__ src,
+                                               int nxy, int neta, float norm_factor) {
+  __shared__ float tile[64][65];
+  src3d_tile(e_gev, src, nxy, neta, norm_factor, blockIdx.x, blockIdx.y, tile);
+}
+
+// Batched: item = (event, xy tile, eta tile).
+__global__ __launch_bounds__(256) void k_src3d_batch(const Event3D *__restrict__ evs,
+                                                     const int4 *__restrict__ items, int n_items) {
+  __shared__ float tile[64][65];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    int4 item = items[it];
+    const Event3D ev = evs[item.x];
+    src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, item.y, item.z, tile);
   }
 }
 
@@ -641,13 +559,6 @@ hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t fsm_launch_stream2d(const Event2D &ev, hipStream_t s) {
-  int tiles_y = (ev.ny + FSM2D_TY - 1) / FSM2D_TY;
-  size_t lds = (size_t)ev.lds_floats4 * sizeof(float4);
-  hipLaunchKernelGGL(k_stream2d_one, dim3(ev.tiles_x * tiles_y), dim3(FSM2D_THREADS), lds, s, ev);
-  return hipGetLastError();
-}
-
 hipError_t fsm_launch_src3d(const Event3D &ev, hipStream_t s) {
   int nxy = ev.nx * ev.ny;
   dim3 grid((nxy + 63) / 64, (ev.neta + 63) / 64);
@@ -675,15 +586,38 @@ hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s) {
 }
 
 hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_stream2d_one,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes2d);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_stream2d_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          bytes2d);
+  hipError_t e = fsm_stream2d_set_lds_limit(bytes2d);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void *)k_stream3d_one, hipFuncAttributeMaxDynamicSharedMemorySize,
                           bytes3d);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void *)k_stream3d_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
                              bytes3d);
+}
+
+// ---- batched launchers: one launch per stage for every event of a batch ----
+static inline int batch_grid(int n_items) { return std::max(1, std::min(n_items, 16384)); }
+
+hipError_t fsm_launch_spline2d_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spline2d_batch, dim3(batch_grid(n)), dim3(256), 0, s, evs, items, n);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_src3d_batch(const Event3D *evs, const int4 *items, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_src3d_batch, dim3(batch_grid(n)), dim3(256), 0, s, evs, items, n);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stream3d_batch, dim3(batch_grid(n)), dim3(FSM3D_THREADS), lds_bytes, s, evs, items, n);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_landau_batch, dim3(batch_grid(n)), dim3(256), 0, s, evs, items, n);
+  return hipGetLastError();
 }

@@ -1,0 +1,434 @@
+// fsm_server.cpp -- request loop with continuous batching (fsm_server_* in
+// include/fsm.h).  No counterpart in the reference, which runs one event per
+// call of run_freestream_milne().
+//
+// Threads and streams.  fsm_server_submit runs on the caller's thread: it
+// finalizes the parameters, checks the shape is supported, reserves the
+// request's HBM footprint against the budget, copies the input to pinned
+// staging and queues an asynchronous upload on the upload stream.  A worker
+// thread owns the launch slots (one non-blocking stream, batch and completion
+// event each).  Whenever a slot is free and requests are queued, it takes up
+// to max_batch_events of them, makes the slot's stream wait for their
+// uploads, and launches them as one batch (one launch per stage); it retires
+// slots whose completion event has fired.  Requests that arrive while
+// batches run are therefore admitted and launched next to the running work.
+// Results stay in HBM until fetched or released.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fsm.h"
+#include "fsm_host.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Exact-size caching pools: events of a serving mix come in a few shapes, so
+// blocks are reused without fragmentation; everything is freed at destroy.
+struct DevicePool {
+  std::multimap<size_t, void *> free_blocks;
+  std::vector<void *> all;
+  void *get(size_t bytes) {
+    auto it = free_blocks.find(bytes);
+    if (it != free_blocks.end()) {
+      void *p = it->second;
+      free_blocks.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    all.push_back(p);
+    return p;
+  }
+  void put(size_t bytes, void *p) {
+    if (p) free_blocks.emplace(bytes, p);
+  }
+  void release_all() {
+    for (void *p : all) (void)hipFree(p);
+    all.clear();
+    free_blocks.clear();
+  }
+};
+
+struct HostPool {
+  std::multimap<size_t, void *> free_blocks;
+  std::vector<void *> all;
+  void *get(size_t bytes) {
+    auto it = free_blocks.find(bytes);
+    if (it != free_blocks.end()) {
+      void *p = it->second;
+      free_blocks.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    all.push_back(p);
+    return p;
+  }
+  void put(size_t bytes, void *p) {
+    if (p) free_blocks.emplace(bytes, p);
+  }
+  void release_all() {
+    for (void *p : all) (void)hipHostFree(p);
+    all.clear();
+    free_blocks.clear();
+  }
+};
+
+enum ReqState { REQ_QUEUED, REQ_RUNNING, REQ_DONE };
+
+struct Request {
+  int64_t id = 0;
+  fsm_params p{};
+  size_t in_bytes = 0, out_bytes = 0, ws_bytes = 0, footprint = 0;
+  float *d_in = nullptr;
+  double *d_out = nullptr;
+  void *ws = nullptr;
+  float *h_in = nullptr;
+  hipEvent_t uploaded = nullptr;
+  ReqState state = REQ_QUEUED;
+  int status = FSM_OK;
+  Clock::time_point t_submit, t_done;
+};
+
+struct Slot {
+  fsm_batch *batch = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<int64_t> ids;
+  bool busy = false;
+  Clock::time_point t_launch;
+};
+
+}  // namespace
+
+struct fsm_server {
+  fsm_ctx *ctx = nullptr;
+  fsm_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::unordered_map<int64_t, Request *> reqs;
+  std::deque<int64_t> queued;
+  std::deque<int64_t> unreported;
+  std::vector<Slot> slots;
+  std::vector<hipEvent_t> event_pool;
+  hipStream_t upload_stream = nullptr;
+  std::mutex pool_mu;  // guards the memory pools (client and worker threads)
+  DevicePool dpool;
+  HostPool hpool;
+  size_t bytes_in_use = 0;
+  int64_t next_id = 0, n_admitted = 0, n_completed = 0, n_launches = 0;
+  bool stop = false;
+  std::thread worker;
+};
+
+namespace {
+
+void free_request(fsm_server *s, Request *r) {
+  {
+    std::lock_guard<std::mutex> g(s->pool_mu);
+    s->dpool.put(r->in_bytes, r->d_in);
+    s->dpool.put(r->out_bytes, r->d_out);
+    s->dpool.put(r->ws_bytes, r->ws);
+    s->hpool.put(r->in_bytes, r->h_in);
+  }
+  if (r->uploaded) s->event_pool.push_back(r->uploaded);  // caller holds s->mu
+  delete r;
+}
+
+// Worker: launch queued requests into free slots, retire finished slots.
+void worker_loop(fsm_server *s) {
+  (void)hipSetDevice(fsm_host::ctx_device(s->ctx));
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    // retire finished launches
+    for (auto &sl : s->slots) {
+      if (!sl.busy) continue;
+      hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      const Clock::time_point now = Clock::now();
+      for (int64_t id : sl.ids) {
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) continue;
+        it->second->state = REQ_DONE;
+        if (q != hipSuccess) it->second->status = FSM_ERR_DEVICE;
+        it->second->t_done = now;
+        s->unreported.push_back(id);
+        ++s->n_completed;
+      }
+      sl.ids.clear();
+      sl.busy = false;
+      s->cv_done.notify_all();
+    }
+    // launch into a free slot
+    Slot *free_slot = nullptr;
+    for (auto &sl : s->slots)
+      if (!sl.busy) {
+        free_slot = &sl;
+        break;
+      }
+    if (free_slot && !s->queued.empty()) {
+      std::vector<Request *> take;
+      while (!s->queued.empty() && (int)take.size() < s->cfg.max_batch_events) {
+        auto it = s->reqs.find(s->queued.front());
+        s->queued.pop_front();
+        if (it == s->reqs.end() || it->second->state != REQ_QUEUED) continue;
+        it->second->state = REQ_RUNNING;
+        take.push_back(it->second);
+      }
+      if (!take.empty()) {
+        Slot &sl = *free_slot;
+        sl.busy = true;
+        sl.ids.clear();
+        for (Request *r : take) sl.ids.push_back(r->id);
+        lk.unlock();
+        int st = fsm_batch_clear(sl.batch);
+        for (Request *r : take) {
+          if (st) break;
+          if (hipStreamWaitEvent(sl.stream, r->uploaded, 0) != hipSuccess) st = FSM_ERR_DEVICE;
+          int i = fsm_host::batch_add_ws(sl.batch, &r->p, r->d_in, r->d_out, r->ws);
+          if (i < 0) st = i;
+        }
+        if (!st) st = fsm_host::batch_launch_now(sl.batch, sl.stream);
+        hipError_t er = hipEventRecord(sl.done, sl.stream);
+        lk.lock();
+        ++s->n_launches;
+        if (st || er != hipSuccess)
+          for (Request *r : take) r->status = st ? st : FSM_ERR_DEVICE;
+        continue;
+      }
+    }
+    bool any_busy = false;
+    for (auto &sl : s->slots) any_busy |= sl.busy;
+    if (s->stop && s->queued.empty() && !any_busy) break;
+    if (any_busy)
+      s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+    else
+      s->cv_work.wait(lk, [s] { return s->stop || !s->queued.empty(); });
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsm_server_create(fsm_ctx *ctx, const fsm_server_config *cfg, fsm_server **out) {
+  if (!ctx || !out) return FSM_ERR_ARG;
+  *out = nullptr;
+  if (hipSetDevice(fsm_host::ctx_device(ctx)) != hipSuccess) return FSM_ERR_DEVICE;
+  fsm_server *s = new (std::nothrow) fsm_server();
+  if (!s) return FSM_ERR_OOM;
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = (size_t)16 << 30;
+  if (s->cfg.max_batch_events <= 0) s->cfg.max_batch_events = 256;
+  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
+  if (s->cfg.max_requests <= 0) s->cfg.max_requests = 4096;
+  int st = FSM_OK;
+  if (hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking) != hipSuccess) st = FSM_ERR_DEVICE;
+  s->slots.resize(s->cfg.max_inflight);
+  for (auto &sl : s->slots) {
+    if (st) break;
+    if (fsm_batch_create(ctx, &sl.batch) != FSM_OK) st = FSM_ERR_OOM;
+    if (!st && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) st = FSM_ERR_DEVICE;
+    if (!st && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) st = FSM_ERR_DEVICE;
+    if (!st) fsm_host::batch_set_async_upload(sl.batch, true);
+  }
+  if (st) {
+    fsm_server_destroy(s);
+    return st;
+  }
+  s->worker = std::thread(worker_loop, s);
+  *out = s;
+  return FSM_OK;
+}
+
+void fsm_server_destroy(fsm_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+    // queued requests are dropped; launched ones finish
+    for (int64_t id : s->queued) {
+      auto it = s->reqs.find(id);
+      if (it != s->reqs.end() && it->second->state == REQ_QUEUED) it->second->state = REQ_DONE;
+    }
+    s->queued.clear();
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(fsm_host::ctx_device(s->ctx));
+  for (auto &sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.batch) fsm_batch_destroy(sl.batch);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  if (s->upload_stream) {
+    (void)hipStreamSynchronize(s->upload_stream);
+    (void)hipStreamDestroy(s->upload_stream);
+  }
+  for (auto &kv : s->reqs) {
+    if (kv.second->uploaded) (void)hipEventDestroy(kv.second->uploaded);
+    delete kv.second;
+  }
+  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+  s->dpool.release_all();
+  s->hpool.release_all();
+  delete s;
+}
+
+int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV, const float *nB, size_t n) {
+  if (!s || !p || !e_GeV) return FSM_ERR_ARG;
+  fsm_params q = *p;
+  int st = fsm_params_finalize(&q, e_GeV, n);
+  if (st) return st;
+  if (q.BARYON || nB) return FSM_ERR_UNSUPPORTED;
+  st = fsm_host::plan_check(s->ctx, &q);
+  if (st) return st;
+  Request *r = new (std::nothrow) Request();
+  if (!r) return FSM_ERR_OOM;
+  r->p = q;
+  r->in_bytes = (size_t)q.DIM * sizeof(float);
+  r->out_bytes = (size_t)q.DIM * FSM_NUM_OUTPUTS * sizeof(double);
+  r->ws_bytes = fsm_host::workspace_bytes(q);
+  r->footprint = r->in_bytes + r->out_bytes + r->ws_bytes;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if ((int)s->reqs.size() >= s->cfg.max_requests ||
+        s->bytes_in_use + r->footprint > s->cfg.hbm_budget_bytes) {
+      delete r;
+      return FSM_ERR_BUSY;
+    }
+    s->bytes_in_use += r->footprint;
+    r->id = s->next_id++;
+    if (!s->event_pool.empty()) {
+      r->uploaded = s->event_pool.back();
+      s->event_pool.pop_back();
+    }
+  }
+  bool ok = true;
+  {
+    std::lock_guard<std::mutex> g(s->pool_mu);
+    r->d_in = (float *)s->dpool.get(r->in_bytes);
+    r->d_out = (double *)s->dpool.get(r->out_bytes);
+    r->ws = s->dpool.get(r->ws_bytes);
+    r->h_in = (float *)s->hpool.get(r->in_bytes);
+    ok = r->d_in && r->d_out && r->ws && r->h_in;
+  }
+  if (ok && !r->uploaded) ok = hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming) == hipSuccess;
+  if (ok) {
+    memcpy(r->h_in, e_GeV, r->in_bytes);
+    ok = hipMemcpyAsync(r->d_in, r->h_in, r->in_bytes, hipMemcpyHostToDevice, s->upload_stream) == hipSuccess &&
+         hipEventRecord(r->uploaded, s->upload_stream) == hipSuccess;
+  }
+  std::lock_guard<std::mutex> g(s->mu);
+  if (!ok) {
+    s->bytes_in_use -= r->footprint;
+    free_request(s, r);
+    return FSM_ERR_OOM;
+  }
+  r->t_submit = Clock::now();
+  r->state = REQ_QUEUED;
+  s->reqs[r->id] = r;
+  s->queued.push_back(r->id);
+  ++s->n_admitted;
+  s->cv_work.notify_one();
+  return r->id;
+}
+
+int64_t fsm_server_poll(fsm_server *s, int timeout_us) {
+  if (!s) return FSM_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->unreported.empty() && timeout_us > 0)
+    s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), [s] { return !s->unreported.empty(); });
+  if (s->unreported.empty()) return -1;
+  int64_t id = s->unreported.front();
+  s->unreported.pop_front();
+  return id;
+}
+
+int fsm_server_result(fsm_server *s, int64_t id, double *latency_us, double **d_out) {
+  if (!s) return FSM_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return FSM_ERR_ARG;
+  Request *r = it->second;
+  if (r->state != REQ_DONE) return FSM_ERR_BUSY;
+  if (latency_us) latency_us[0] = std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count();
+  if (d_out) *d_out = r->d_out;
+  return r->status;
+}
+
+int fsm_server_release(fsm_server *s, int64_t id) {
+  if (!s) return FSM_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return FSM_ERR_ARG;
+  Request *r = it->second;
+  if (r->state != REQ_DONE) return FSM_ERR_BUSY;
+  s->bytes_in_use -= r->footprint;
+  s->reqs.erase(it);
+  free_request(s, r);
+  return FSM_OK;
+}
+
+int fsm_server_fetch(fsm_server *s, int64_t id, double *const out[FSM_NUM_OUTPUTS]) {
+  if (!s || !out) return FSM_ERR_ARG;
+  double *d_out = nullptr;
+  int st = fsm_server_result(s, id, nullptr, &d_out);
+  if (st == FSM_ERR_BUSY || st == FSM_ERR_ARG) return st;
+  if (st == FSM_OK) {
+    size_t D;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      D = (size_t)s->reqs[id]->p.DIM;
+    }
+    (void)hipSetDevice(fsm_host::ctx_device(s->ctx));
+    for (int k = 0; k < FSM_NUM_OUTPUTS && st == FSM_OK; ++k)
+      if (out[k] && hipMemcpy(out[k], d_out + k * D, D * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        st = FSM_ERR_DEVICE;
+  }
+  int rs = fsm_server_release(s, id);
+  return st ? st : rs;
+}
+
+int fsm_server_cancel(fsm_server *s, int64_t id) {
+  if (!s) return FSM_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return FSM_ERR_ARG;
+  Request *r = it->second;
+  if (r->state != REQ_QUEUED) return FSM_ERR_BUSY;
+  auto q = std::find(s->queued.begin(), s->queued.end(), id);
+  if (q != s->queued.end()) s->queued.erase(q);
+  r->state = REQ_DONE;
+  r->status = FSM_ERR_CANCELLED;
+  r->t_done = Clock::now();
+  s->unreported.push_back(id);
+  s->cv_done.notify_all();
+  return FSM_OK;
+}
+
+int fsm_server_stats(fsm_server *s, int64_t *admitted, int64_t *completed, int64_t *launches,
+                     size_t *bytes_in_use) {
+  if (!s) return FSM_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (admitted) *admitted = s->n_admitted;
+  if (completed) *completed = s->n_completed;
+  if (launches) *launches = s->n_launches;
+  if (bytes_in_use) *bytes_in_use = s->bytes_in_use;
+  return FSM_OK;
+}
+
+}  // extern "C"

@@ -85,6 +85,13 @@ class Params(ctypes.Structure):
         return "Params(%s)" % ", ".join("%s=%r" % kv for kv in self.as_dict().items())
 
 
+class ServerConfig(ctypes.Structure):
+    """fsm_server_config (include/fsm.h)."""
+
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_events", ctypes.c_int),
+                ("max_inflight", ctypes.c_int), ("max_requests", ctypes.c_int)]
+
+
 class FsmError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -126,6 +133,28 @@ def _lib():
         lib.fsm_enable_timing.argtypes = [vp, ctypes.c_int]
         lib.fsm_last_kernel_ms.argtypes = [vp, ctypes.c_int, fp]
         lib.fsm_kernel_time_totals.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+        lib.fsm_batch_create.argtypes = [vp, ctypes.POINTER(vp)]
+        lib.fsm_batch_destroy.argtypes = [vp]
+        lib.fsm_batch_destroy.restype = None
+        lib.fsm_batch_add.argtypes = [vp, P, vp, vp]
+        lib.fsm_batch_clear.argtypes = [vp]
+        lib.fsm_batch_size.argtypes = [vp]
+        lib.fsm_batch_set_graph.argtypes = [vp, ctypes.c_int]
+        lib.fsm_batch_run.argtypes = [vp, vp]
+        lib.fsm_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        lib.fsm_server_destroy.argtypes = [vp]
+        lib.fsm_server_destroy.restype = None
+        lib.fsm_server_submit.argtypes = [vp, P, fp, fp, ctypes.c_size_t]
+        lib.fsm_server_submit.restype = ctypes.c_int64
+        lib.fsm_server_poll.argtypes = [vp, ctypes.c_int]
+        lib.fsm_server_poll.restype = ctypes.c_int64
+        lib.fsm_server_result.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_void_p)]
+        lib.fsm_server_fetch.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]
+        lib.fsm_server_release.argtypes = [vp, ctypes.c_int64]
+        lib.fsm_server_cancel.argtypes = [vp, ctypes.c_int64]
+        lib.fsm_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_size_t)]
         _LIB = lib
     return _LIB
 
@@ -270,6 +299,124 @@ class Context:
         v = ctypes.c_float()
         _check(_lib().fsm_last_kernel_ms(self._ctx, which, ctypes.byref(v)), "fsm_last_kernel_ms")
         return v.value
+
+
+class Batch:
+    """Many independent events run with one launch per stage (fsm_batch_*).
+
+    ``add(params_finalized, d_e, d_out)`` takes device pointers (input DIM
+    floats in GeV/fm^3, output 17*DIM doubles); ``run(stream)`` replays a HIP
+    graph of the whole batch while its composition is unchanged."""
+
+    def __init__(self, ctx: Context):
+        self._ctx = ctx
+        self._b = ctypes.c_void_p()
+        self._params = []  # keep the structs alive while referenced
+        _check(_lib().fsm_batch_create(ctx._ctx, ctypes.byref(self._b)), "fsm_batch_create")
+
+    def add(self, p: Params, d_e: int, d_out: int) -> int:
+        q = p.copy()
+        st = _lib().fsm_batch_add(self._b, ctypes.byref(q), d_e, d_out)
+        if st < 0:
+            raise FsmError(st, "fsm_batch_add")
+        self._params.append(q)
+        return st
+
+    def clear(self):
+        _check(_lib().fsm_batch_clear(self._b), "fsm_batch_clear")
+        self._params = []
+
+    def __len__(self):
+        return _lib().fsm_batch_size(self._b)
+
+    def set_graph(self, on: bool):
+        _check(_lib().fsm_batch_set_graph(self._b, 1 if on else 0), "fsm_batch_set_graph")
+
+    def run(self, stream=None):
+        _check(_lib().fsm_batch_run(self._b, _stream_handle(stream)), "fsm_batch_run")
+
+    def close(self):
+        if self._b:
+            _lib().fsm_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (fsm_server_*).
+
+    ``submit(params, e)`` admits one event (host input, GeV/fm^3) and returns a
+    request id, or None when the request slots / HBM budget are full
+    (FSM_ERR_BUSY).  ``poll(timeout_us)`` returns the id of a completed
+    request or None.  Results stay in HBM: ``result(id)`` gives (status,
+    latency_us, device pointer of the 17*DIM outputs); ``fetch(id)`` copies
+    them to host and releases the request."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, max_batch_events: int = 0,
+                 max_inflight: int = 0, max_requests: int = 0):
+        self._ctx = ctx
+        cfg = ServerConfig(hbm_budget_bytes, max_batch_events, max_inflight, max_requests)
+        self._s = ctypes.c_void_p()
+        _check(_lib().fsm_server_create(ctx._ctx, ctypes.byref(cfg), ctypes.byref(self._s)), "fsm_server_create")
+        self._dims = {}
+
+    def submit(self, p: Params, e: np.ndarray):
+        e = np.ascontiguousarray(e, dtype=np.float32).ravel()
+        q = p.copy()
+        rid = _lib().fsm_server_submit(self._s, ctypes.byref(q), _fptr(e), None, e.size)
+        if rid == FSM_ERR_BUSY:
+            return None
+        if rid < 0:
+            raise FsmError(int(rid), "fsm_server_submit")
+        self._dims[rid] = e.size
+        return rid
+
+    def poll(self, timeout_us: int = 0):
+        rid = _lib().fsm_server_poll(self._s, timeout_us)
+        return None if rid < 0 else rid
+
+    def result(self, rid: int):
+        lat = ctypes.c_double()
+        ptr = ctypes.c_void_p()
+        st = _lib().fsm_server_result(self._s, rid, ctypes.byref(lat), ctypes.byref(ptr))
+        return st, lat.value, ptr.value
+
+    def fetch(self, rid: int):
+        n = self._dims.pop(rid)
+        outs = [np.empty(n, dtype=np.float64) for _ in range(NUM_OUTPUTS)]
+        arr = (ctypes.c_void_p * NUM_OUTPUTS)(*[o.ctypes.data for o in outs])
+        _check(_lib().fsm_server_fetch(self._s, rid, arr), "fsm_server_fetch")
+        return dict(zip(OUTPUT_NAMES, outs))
+
+    def release(self, rid: int):
+        self._dims.pop(rid, None)
+        _check(_lib().fsm_server_release(self._s, rid), "fsm_server_release")
+
+    def cancel(self, rid: int) -> int:
+        return _lib().fsm_server_cancel(self._s, rid)
+
+    def stats(self):
+        a, c, l = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        b = ctypes.c_size_t()
+        _check(_lib().fsm_server_stats(self._s, ctypes.byref(a), ctypes.byref(c), ctypes.byref(l), ctypes.byref(b)),
+               "fsm_server_stats")
+        return {"admitted": a.value, "completed": c.value, "launches": l.value, "bytes_in_use": b.value}
+
+    def close(self):
+        if self._s:
+            _lib().fsm_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FreestreamMilne:

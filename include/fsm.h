@@ -188,6 +188,74 @@ int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T,
                       float *d_e, float *d_u, float *d_p, float *d_Pi,
                       float *d_pi, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Batches: many independent events (any mix of shapes, 2-D and 3-D) run with
+ * one launch per stage over a work list of (event, tile) items, captured in a
+ * HIP graph and replayed while the batch is unchanged.  No reference
+ * counterpart: the reference runs one event per process call.
+ *
+ * fsm_batch_add takes finalized params and device buffers (input DIM floats
+ * in GeV/fm^3, output 17*DIM doubles, as fsm_run_device) and returns the
+ * event's index (>= 0) or a negative status; the pointers must stay valid
+ * until fsm_batch_clear.  Workspace is allocated by the batch on its first
+ * run after a change. */
+typedef struct fsm_batch fsm_batch;
+int fsm_batch_create(fsm_ctx *ctx, fsm_batch **out);
+void fsm_batch_destroy(fsm_batch *b);
+int fsm_batch_add(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, double *d_out);
+int fsm_batch_clear(fsm_batch *b);
+int fsm_batch_size(const fsm_batch *b);
+/* use_graph: 1 (default) capture and replay a HIP graph; 0 launch directly
+ * (stage timing with fsm_enable_timing is recorded only in this mode). */
+int fsm_batch_set_graph(fsm_batch *b, int use_graph);
+int fsm_batch_run(fsm_batch *b, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Serving: a request loop with continuous batching.  No reference
+ * counterpart (the reference runs one event per process call).
+ *
+ * A worker thread owned by the server launches whatever requests have been
+ * admitted whenever a launch slot is free, as one batch (one launch per
+ * stage); requests that arrive while earlier batches run are admitted as
+ * request slots and HBM budget allow, and their uploads and launches overlap
+ * the running work.  Inputs are copied to the device at submit; results stay
+ * in HBM until fetched or released. */
+typedef struct fsm_server fsm_server;
+typedef struct fsm_server_config {
+  size_t hbm_budget_bytes; /* device memory for admitted requests (0: 16 GiB) */
+  int max_batch_events;    /* events per launch (0: 256) */
+  int max_inflight;        /* launches in flight (0: 2) */
+  int max_requests;        /* admitted + completed-unreleased requests (0: 4096) */
+} fsm_server_config;
+
+int fsm_server_create(fsm_ctx *ctx, const fsm_server_config *cfg, fsm_server **out);
+/* Waits for launched work, drops queued requests, frees everything. */
+void fsm_server_destroy(fsm_server *s);
+/* Admits one event: p as read (it is finalized with the input, as fsm_run
+ * does), e_GeV (n floats, GeV/fm^3) and, for BARYON = 1, nB (n floats;
+ * NULL otherwise).  Returns a request id >= 0, FSM_ERR_BUSY when the request
+ * slots or the HBM budget are exhausted (nothing admitted), or another
+ * negative status. */
+int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV, const float *nB, size_t n);
+/* Id of a completed request not yet reported by poll, or -1 if none completes
+ * within timeout_us microseconds (0: do not wait). */
+int64_t fsm_server_poll(fsm_server *s, int timeout_us);
+/* Completed request: status of its run, submit-to-completion latency in
+ * microseconds (may be NULL) and its device output (17*DIM doubles, may be
+ * NULL), valid until released. */
+int fsm_server_result(fsm_server *s, int64_t id, double *latency_us, double **d_out);
+/* Copies a completed request's 17 output vectors (DIM doubles each) to host
+ * arrays, then releases it. */
+int fsm_server_fetch(fsm_server *s, int64_t id, double *const out[FSM_NUM_OUTPUTS]);
+/* Releases a completed request (its HBM returns to the budget). */
+int fsm_server_release(fsm_server *s, int64_t id);
+/* Cancels a request that has not been launched: FSM_OK (its result status is
+ * FSM_ERR_CANCELLED), or FSM_ERR_BUSY if it is already running or done. */
+int fsm_server_cancel(fsm_server *s, int64_t id);
+/* Counters: requests admitted, completed, launches, bytes of HBM in use. */
+int fsm_server_stats(fsm_server *s, int64_t *admitted, int64_t *completed, int64_t *launches,
+                     size_t *bytes_in_use);
+
 /* Waits for all work the context queued on `stream`. */
 int fsm_sync(fsm_ctx *ctx, void *stream);
 

@@ -1,0 +1,113 @@
+"""Serving loop (fsm_server_*): continuous batching.  Each result must be the
+same alone and in the mix: every request's 17 output vectors are compared
+with the same event served alone (one request in flight), and must be
+identical.  Also: admission refuses work beyond the HBM budget, a queued
+request can be cancelled, and requests arriving while earlier launches run are
+launched together."""
+import time
+
+import numpy as np
+import pytest
+
+import fsm_amd
+import fsm_amd.workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(n, seed):
+    evs = W.batch_mix(n, seed=seed)
+    evs.append(W.small_3d())
+    return evs
+
+
+def _same(a, b, scale):
+    # the same kernels run alone and in the mix, so the results are identical
+    for name in fsm_amd.OUTPUT_NAMES:
+        assert np.array_equal(a[name], b[name]), name
+
+
+def _alone(ctx, evs):
+    """Each event served by itself: one request in flight at a time."""
+    srv = fsm_amd.Server(ctx, max_batch_events=1, max_inflight=1)
+    outs = []
+    for p, e in evs:
+        rid = srv.submit(p, e)
+        assert srv.poll(2000000) == rid
+        outs.append(srv.fetch(rid))
+    srv.close()
+    return outs
+
+
+def test_results_alone_and_in_the_mix(gpu_ctx):
+    evs = _mix(12, seed=21)
+    alone = _alone(gpu_ctx, evs)
+    srv = fsm_amd.Server(gpu_ctx, max_batch_events=5, max_inflight=2)
+    ids = []
+    for p, e in evs:
+        rid = srv.submit(p, e)
+        assert rid is not None
+        ids.append(rid)
+    done = set()
+    t0 = time.time()
+    while len(done) < len(ids):
+        rid = srv.poll(200000)
+        assert rid is not None, "no completion within 0.2 s"
+        done.add(rid)
+        assert time.time() - t0 < 60
+    for rid, ref in zip(ids, alone):
+        st, lat, ptr = srv.result(rid)
+        assert st == 0 and lat > 0 and ptr
+        got = srv.fetch(rid)
+        _same(got, ref, np.abs(ref["e"]).max())
+    stats = srv.stats()
+    assert stats["completed"] == len(ids) and stats["launches"] >= 3 and stats["bytes_in_use"] == 0
+    srv.close()
+
+
+def test_arrivals_during_running_work_are_batched(gpu_ctx):
+    p, e = W.config("C1")
+    srv = fsm_amd.Server(gpu_ctx, max_batch_events=64, max_inflight=2)
+    ids = [srv.submit(p, e) for _ in range(40)]
+    got = 0
+    while got < len(ids):
+        if srv.poll(500000) is not None:
+            got += 1
+    st = srv.stats()
+    # 40 requests arriving faster than one event runs are launched in fewer
+    # launches than requests
+    assert st["launches"] < len(ids)
+    ref = _alone(gpu_ctx, [(p, e)])[0]
+    for rid in ids:
+        _same(srv.fetch(rid), ref, np.abs(ref["e"]).max())
+    srv.close()
+
+
+def test_budget_admission_and_cancel(gpu_ctx):
+    p, e = W.config("C1")
+    D = p.DIM_X * p.DIM_Y
+    per_event = D * (4 + 17 * 8)  # input + outputs, workspace comes on top
+    srv = fsm_amd.Server(gpu_ctx, hbm_budget_bytes=3 * per_event, max_batch_events=1, max_inflight=1)
+    a = srv.submit(p, e)
+    b = srv.submit(p, e)
+    assert a is not None
+    # the third cannot fit the budget once workspace is counted
+    c = srv.submit(p, e)
+    assert c is None or b is None
+    # cancel whatever is still queued
+    if b is not None:
+        r = srv.cancel(b)
+        assert r in (0, fsm_amd.FSM_ERR_BUSY)
+    seen = set()
+    while len(seen) < (2 if b is not None else 1):
+        rid = srv.poll(500000)
+        assert rid is not None
+        seen.add(rid)
+    if b is not None:
+        st, _, _ = srv.result(b)
+        assert st in (0, fsm_amd.FSM_ERR_CANCELLED)
+        srv.release(b)
+    assert srv.result(a)[0] == 0
+    srv.release(a)
+    assert srv.stats()["bytes_in_use"] == 0
+    srv.close()
