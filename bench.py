@@ -1,15 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the free-streaming + Landau-matching step on one MI355X.
 
-Workload (default): the 2-D lumpy event -- 261 x 261 cells, DX = DY = 0.06 fm,
-200 Gaussian hotspots, DIM_PHIP = 200, TAU0 = 0.1, DTAU = 1.0 fm/c, conformal
-EoS, viscous matching -- streamed and Landau-matched on the GPU with the
-event already resident in HBM.  One step = one batch of `--batch` independent
-events of that shape (different seeds).  `value` is events/s over the timed
-steps.  The line also carries single-event latency (p50/p99), the 131^2 x 101
-3-D event rate, the dominant kernel's roofline position (HIP events on the
-launch stream, summed over the timed region) and the reference-algorithm CPU
-baseline (the oracle's float path, OpenMP on this host's cores).
+Workload (the line's `value`): the 2-D lumpy event -- 261 x 261 cells,
+DX = DY = 0.06 fm, 200 Gaussian hotspots, DIM_PHIP = 200, TAU0 = 0.1,
+DTAU = 1.0 fm/c, conformal EoS, viscous matching -- with the events already
+resident in HBM.  One step = one batch of `--batch` independent events of that
+shape (different seeds) run with one launch per stage, replayed from a HIP
+graph.  `value` is events/s over the timed steps.
+
+The line also carries: the plain path (one launch sequence per event, no
+graph) on the same events; single-event latency (p50/p99, device-resident);
+the 131^2 x 101 3-D event; a short serving run (Poisson arrivals through the
+request loop, host inputs, results left in HBM) with p50/p99 latency; the
+dominant kernel's roofline position (HIP events on the launch stream); and the
+reference-algorithm CPU baseline (the oracle's float path, OpenMP on this
+host's cores).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -28,7 +33,6 @@ for p in (os.path.join(ROOT, "freestream-milne_amd"), os.path.join(ROOT, "oracle
     if p not in sys.path:
         sys.path.insert(0, p)
 
-HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_SAMPLE_2D = 52  # 16 spline taps * 2 + 10 quadrature rows * 2
 FLOP_PER_SAMPLE_3D = 36  # 8 trilinear taps * 2 + 10 * 2
@@ -39,13 +43,73 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--latency-samples", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=10)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     return ap.parse_args()
+
+
+def timed(fn, steps, sync):
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def serve_run(fsm_amd, W, ctx, seconds, rate, seed=5):
+    """Poisson arrivals of C1-shape and mixed 2-D events through the server;
+    latency = submit to results in HBM."""
+    rng = np.random.default_rng(seed)
+    pool = [W.config("C1")] + W.batch_mix(15, seed=seed)
+    srv = fsm_amd.Server(ctx, max_batch_events=128, max_inflight=2)
+    for p, e in pool[:4]:  # build plans outside the timed run
+        rid = srv.submit(p, e)
+        while srv.poll(100000) != rid:
+            pass
+        srv.release(rid)
+    lat, done, busy = [], 0, 0
+    t0 = time.perf_counter()
+    t_next = t0
+    submitted = []
+    while True:
+        now = time.perf_counter()
+        if now - t0 < seconds and now >= t_next:
+            p, e = pool[rng.integers(len(pool))]
+            rid = srv.submit(p, e)
+            if rid is None:
+                busy += 1
+            else:
+                submitted.append(rid)
+            t_next += rng.exponential(1.0 / rate)
+            continue
+        rid = srv.poll(50)
+        if rid is not None:
+            st, l_us, _ = srv.result(rid)
+            lat.append(l_us / 1e3)
+            srv.release(rid)
+            done += 1
+        elif now - t0 >= seconds and done >= len(submitted):
+            break
+        if now - t0 > seconds + 30:
+            break
+    el = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    lat = np.array(lat) if lat else np.zeros(1)
+    return {"workload": "Poisson arrivals, rate %.0f/s for %.1f s: C1 events and 2-D events of mixed size "
+                        "(side 101-301, DIM_PHIP 48-200), host inputs, results kept in HBM" % (rate, seconds),
+            "completed": done, "refused_busy": busy, "events_per_s": round(done / el, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                           "p99": round(float(np.percentile(lat, 99)), 4)},
+            "launches": stats["launches"]}
 
 
 def main():
@@ -60,9 +124,10 @@ def main():
 
     dev = torch.device("cuda:0")
     ctx = fsm_amd.Context(0)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()
+    sync = torch.cuda.synchronize
 
-    # ---- C1 batch ---------------------------------------------------------
+    # ---- C1 batch -----------------------------------------------------------
     B = args.batch
     p1, e1 = W.config("C1")
     q1 = fsm_amd.params_finalize(p1, e1)
@@ -70,36 +135,41 @@ def main():
     evs = [e1] + [W.config("C1", seed=100 + k)[1] for k in range(B - 1)]
     d_in = torch.from_numpy(np.stack(evs).astype(np.float32)).to(dev)
     d_out = torch.empty((B, 17, D), dtype=torch.float64, device=dev)
+    batch = fsm_amd.Batch(ctx)
+    for b in range(B):
+        batch.add(q1, d_in[b].data_ptr(), d_out[b].data_ptr())
 
-    def step():
+    def step_batch():
+        batch.run(stream)
+
+    def step_plain():
         for b in range(B):
             ctx.run_device(q1, d_in[b].data_ptr(), d_out[b].data_ptr(), stream)
 
     for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+        step_batch()
+        step_plain()
+    el = timed(step_batch, args.steps, sync)
+    value = B * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+    el_plain = timed(step_plain, max(1, args.steps // 2), sync)
+    plain = B * max(1, args.steps // 2) / el_plain
+
+    # per-stage kernel times: the same batch launched directly with HIP events
+    # between stages on the launch stream
+    batch.set_graph(False)
     ctx.enable_timing(True)
-    ctx.kernel_time_totals()  # reset
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    ctx.kernel_time_totals()
+    timed(step_batch, args.steps, sync)
     ktot, ncalls = ctx.kernel_time_totals()
     ctx.enable_timing(False)
-    elapsed = t1 - t0
-    events = B * args.steps
-    value = events / elapsed
-    ms_per_step = elapsed * 1e3 / args.steps
-
-    # dominant kernel: stream + quadrature (stage index 1)
-    k_ms = ktot[1] / max(ncalls, 1)
-    samples = D * q1.DIM_PHIP * q1.DIM_RAP
+    batch.set_graph(True)
+    k_ms = ktot[1] / max(ncalls, 1)  # stream + quadrature launch (B events)
+    samples = D * q1.DIM_PHIP * q1.DIM_RAP * B
     flops = FLOP_PER_SAMPLE_2D * samples
     achieved = flops / (k_ms * 1e-3) / 1e12
     roofline = {
-        "kernel": "k_stream2d_one",
+        "kernel": "k_stream2d_batch",
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
@@ -108,22 +178,23 @@ def main():
         "traffic": None,
         "avg_kernel_ms": round(k_ms, 5),
         "flop_per_launch": flops,
-        "note": "FP32 VALU kernel: 52 algorithmic FLOP per (cell, phi) sample; 157.3 TFLOP/s is "
-                "MI355X's FP32 peak",
+        "note": "FP32 VALU kernel (no MFMA): 52 algorithmic FLOP per (cell, phi) sample, %d events per launch; "
+                "157.3 TFLOP/s is MI355X's FP32 peak" % B,
     }
-    stage_ms = {"spline": ktot[0] / max(ncalls, 1), "stream": k_ms, "landau": ktot[2] / max(ncalls, 1)}
+    stage_ms = {"spline": ktot[0] / max(ncalls, 1) / B, "stream": k_ms / B, "landau": ktot[2] / max(ncalls, 1) / B}
 
-    # ---- single-event latency ---------------------------------------------
+    # ---- single-event latency (device-resident) ------------------------------
     lat = []
     for i in range(args.latency_samples):
-        torch.cuda.synchronize()
+        sync()
         a = time.perf_counter()
         ctx.run_device(q1, d_in[i % B].data_ptr(), d_out[i % B].data_ptr(), stream)
-        torch.cuda.synchronize()
+        stream.synchronize()
         lat.append((time.perf_counter() - a) * 1e3)
     lat = np.array(lat)
+    del d_out
 
-    # ---- 3-D event ----------------------------------------------------------
+    # ---- 3-D event ---------------------------------------------------------------
     c2 = None
     if not args.no_3d:
         p2, e2 = W.config("C2")
@@ -131,23 +202,27 @@ def main():
         d2 = torch.from_numpy(e2).to(dev)
         o2 = torch.empty(17 * q2.DIM, dtype=torch.float64, device=dev)
         ctx.run_device(q2, d2.data_ptr(), o2.data_ptr(), stream)
-        torch.cuda.synchronize()
+        sync()
         ctx.enable_timing(True)
         ctx.kernel_time_totals()
-        a = time.perf_counter()
-        for _ in range(args.c2_steps):
-            ctx.run_device(q2, d2.data_ptr(), o2.data_ptr(), stream)
-        torch.cuda.synchronize()
-        b = time.perf_counter()
+        el2 = timed(lambda: ctx.run_device(q2, d2.data_ptr(), o2.data_ptr(), stream), args.c2_steps, sync)
         kt2, n2 = ctx.kernel_time_totals()
         ctx.enable_timing(False)
-        c2_ms = (b - a) * 1e3 / args.c2_steps
-        c2 = {"workload": "C2 3-D 131x131x101, DIM_RAP=21, DIM_PHIP=51, DTAU=0.8",
+        c2_ms = el2 * 1e3 / args.c2_steps
+        samples3 = q2.DIM * q2.DIM_RAP * q2.DIM_PHIP
+        c2 = {"workload": "3-D 131x131x101, DIM_RAP=21, DIM_PHIP=48->51, DTAU=0.8, SIGMA=1",
               "events_per_s": round(1e3 / c2_ms, 2), "ms_per_event": round(c2_ms, 4),
-              "stage_ms": {"src": kt2[0] / n2, "stream": kt2[1] / n2, "landau": kt2[2] / n2}}
+              "stage_ms": {"src": round(kt2[0] / n2, 4), "stream": round(kt2[1] / n2, 4),
+                           "landau": round(kt2[2] / n2, 4)},
+              "stream_tflops_reference_samples": round(FLOP_PER_SAMPLE_3D * samples3 / (kt2[1] / n2 * 1e-3) / 1e12, 3)}
         del d2, o2
 
-    # ---- CPU baseline (oracle float path = reference algorithm) ------------
+    # ---- serving -----------------------------------------------------------------
+    serve = None
+    if not args.no_serve:
+        serve = serve_run(fsm_amd, W, ctx, args.serve_seconds, args.serve_rate)
+
+    # ---- CPU baseline (oracle float path = the reference algorithm) --------------
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -178,17 +253,21 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "C1 2-D lumpy event 261x261, DX=0.06, 200 hotspots, DIM_PHIP=200, DTAU=1.0; "
-                               "batch of %d events per step, device-resident" % B,
-                   "batch": B, "path": "fp32 per-event launches"},
+        "config": {"workload": "2-D lumpy event 261x261, DX=0.06, 200 hotspots, DIM_PHIP=200, DTAU=1.0; "
+                               "%d events per step, device-resident, batched launch replayed from a HIP graph" % B,
+                   "batch": B, "path": "fp32"},
+        "plain_fp32_events_per_s": round(plain, 3),
+        "batched_over_plain": round(value / plain, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4)},
-        "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "stage_ms_per_event": {k: round(v, 5) for k, v in stage_ms.items()},
         "c2_3d": c2,
+        "serving": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    batch.close()
     ctx.close()
 
 

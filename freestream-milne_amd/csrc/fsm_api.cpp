@@ -792,14 +792,10 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       const int stx = (p.DIM_X + FSMSP_TX - 1) / FSMSP_TX, sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
       for (int x = 0; x < stx; ++x)
         for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
-      // the batch stream kernel is variant 0 for every event
-      d.variant = 0;
-      d.tiles_x = (p.DIM_X + fsm_stream2d_tile_w(0) - 1) / fsm_stream2d_tile_w(0);
-      d.tiles_y = (p.DIM_Y + fsm_stream2d_tile_h(0) - 1) / fsm_stream2d_tile_h(0);
-      e2.back() = d;
+      // each event keeps its plan's tile shape (identical bits alone and batched)
       for (int y = 0; y < d.tiles_y; ++y)
         for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
-      b->lds2 = std::max(b->lds2, fsm_stream2d_lds_bytes(0, d.lds_floats4));
+      b->lds2 = std::max(b->lds2, fsm_stream2d_lds_bytes(d.variant, d.lds_floats4));
       EventLandau L = landau_event(p, d.T, 0);
       L.out = ev.d_out;
       el.push_back(L);

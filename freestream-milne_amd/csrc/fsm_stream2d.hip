@@ -179,14 +179,21 @@ __global__ __launch_bounds__(256) void k_stream2d_one(Event2D ev) {
   stream2d_tile_v2<M, JS>(ev, blockIdx.x % ev.tiles_x, blockIdx.x / ev.tiles_x, lds4);
 }
 
-template <int M, int JS>
+// Batched: item = (event, tile x, tile y); each event keeps the tile shape
+// (variant) its plan chose, so an event gives the same bits alone and in a
+// batch.  The variant is uniform per item, so the switch does not diverge.
 __global__ __launch_bounds__(256) void k_stream2d_batch(const Event2D *__restrict__ evs,
                                                         const int4 *__restrict__ items, int n_items) {
   extern __shared__ float4 lds4[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int4 item = items[it];
     const Event2D ev = evs[item.x];
-    stream2d_tile_v2<M, JS>(ev, item.y, item.z, lds4);
+    switch (ev.variant) {
+      case 1: stream2d_tile_v2<4, 4>(ev, item.y, item.z, lds4); break;
+      case 2: stream2d_tile_v2<2, 4>(ev, item.y, item.z, lds4); break;
+      default: stream2d_tile_v2<4, 1>(ev, item.y, item.z, lds4); break;
+    }
+    __syncthreads();  // all waves leave the tile before the next item reuses LDS
   }
 }
 
@@ -224,13 +231,13 @@ hipError_t fsm_launch_stream2d(const Event2D &ev, hipStream_t s) {
 hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int grid = n < 16384 ? n : 16384;
-  hipLaunchKernelGGL((k_stream2d_batch<4, 1>), dim3(grid), dim3(256), lds_bytes, s, evs, items, n);
+  hipLaunchKernelGGL(k_stream2d_batch, dim3(grid), dim3(256), lds_bytes, s, evs, items, n);
   return hipGetLastError();
 }
 
 hipError_t fsm_stream2d_set_lds_limit(int bytes) {
   const void *fns[] = {(const void *)k_stream2d_one<4, 1>, (const void *)k_stream2d_one<4, 4>,
-                       (const void *)k_stream2d_one<2, 4>, (const void *)k_stream2d_batch<4, 1>};
+                       (const void *)k_stream2d_one<2, 4>, (const void *)k_stream2d_batch};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;

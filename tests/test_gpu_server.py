@@ -68,6 +68,11 @@ def test_results_alone_and_in_the_mix(gpu_ctx):
 def test_arrivals_during_running_work_are_batched(gpu_ctx):
     p, e = W.config("C1")
     srv = fsm_amd.Server(gpu_ctx, max_batch_events=64, max_inflight=2)
+    # first round fills the memory pools (the first allocations are slow)
+    warm = [srv.submit(p, e) for _ in range(40)]
+    for _ in warm:
+        srv.release(srv.poll(2000000))
+    launches0 = srv.stats()["launches"]
     ids = [srv.submit(p, e) for _ in range(40)]
     got = 0
     while got < len(ids):
@@ -76,7 +81,7 @@ def test_arrivals_during_running_work_are_batched(gpu_ctx):
     st = srv.stats()
     # 40 requests arriving faster than one event runs are launched in fewer
     # launches than requests
-    assert st["launches"] < len(ids)
+    assert st["launches"] - launches0 < len(ids)
     ref = _alone(gpu_ctx, [(p, e)])[0]
     for rid in ids:
         _same(srv.fetch(rid), ref, np.abs(ref["e"]).max())
