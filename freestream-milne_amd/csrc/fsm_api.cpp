@@ -236,14 +236,21 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
       sg.bx1 = oxmax + 1;
       sg.by0 = oymin;
       sg.by1 = oymax + 1;
-      // box of the widest tile shape (variant 0)
-      int need = (fsm_stream2d_tile_w(0) + sg.bx1 - sg.bx0) * (fsm_stream2d_tile_h(0) + sg.by1 - sg.by0);
+      // knot box of the widest tile shape (variant 0) + the group's stencils
+      int need = (fsm_stream2d_tile_w(0) + sg.bx1 - sg.bx0) * (fsm_stream2d_tile_h(0) + sg.by1 - sg.by0) +
+                 (int)(sizeof(Stencil2D) / sizeof(float4)) * (j1 - j0);
       lds_max = std::max(lds_max, need);
       groups.push_back(sg);
     }
     if (lds_max <= budget4 || G >= nphi) break;
   }
   if (fsm_stream2d_lds_bytes(1, lds_max) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
+  // Within a group the order of the samples is free (it only changes the
+  // summation order): sort by knot offset so equal offsets are adjacent.
+  for (const StencilGroup &sg : groups)
+    std::stable_sort(st.begin() + sg.j0, st.begin() + sg.j1, [](const Stencil2D &a, const Stencil2D &b) {
+      return a.ox != b.ox ? a.ox < b.ox : a.oy < b.oy;
+    });
   FSM_TRY(pl.gx.upload(gxw));
   FSM_TRY(pl.gy.upload(gyw));
   FSM_TRY(pl.st.upload(st));
@@ -463,7 +470,7 @@ int stage_a(fsm_ctx *ctx, Plan &pl, const fsm_params &p, const float *d_e, float
   return FSM_OK;
 }
 
-EventLandau landau_event(const fsm_params &p, const float *T, int eta_major) {
+EventLandau landau_event(const fsm_params &p, const float *T, int eta_major, int mixed) {
   EventLandau L;
   memset(&L, 0, sizeof(L));
   L.dim = p.DIM;
@@ -473,6 +480,7 @@ EventLandau landau_event(const fsm_params &p, const float *T, int eta_major) {
   L.eos_type = p.EOS_TYPE;
   L.viscous = p.VISCOUS_MATCHING;
   L.fill_outputs = (p.OUTPUTFORMAT == 2 || p.OUTPUTFORMAT == 3) ? 1 : 0;
+  L.mixed = mixed;
   L.tau = p.TAU;
   L.T = T;
   L.t_plane = p.DIM;
@@ -499,7 +507,7 @@ int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double 
   int em = 0;
   st = stage_a(ctx, *pl, *p, d_e, T_scratch, s, &T, &em);
   if (st) return st;
-  EventLandau L = landau_event(*p, T, em);
+  EventLandau L = landau_event(*p, T, em, ctx->path == FSM_PATH_MIXED);
   L.out = d_out;
   FSM_TRY(fsm_launch_landau(L, s));
   tmark(ctx, 3, s);
@@ -557,7 +565,7 @@ void fsm_destroy(fsm_ctx *ctx) {
 
 int fsm_set_path(fsm_ctx *ctx, int path) {
   if (!ctx) return FSM_ERR_ARG;
-  if (path != FSM_PATH_FP32) return FSM_ERR_UNSUPPORTED;
+  if (path != FSM_PATH_FP32 && path != FSM_PATH_MIXED) return FSM_ERR_ARG;
   ctx->path = path;
   return FSM_OK;
 }
@@ -634,7 +642,7 @@ int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T, float
   int st = check_supported(*p);
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
-  EventLandau L = landau_event(*p, d_T, 0);
+  EventLandau L = landau_event(*p, d_T, 0, ctx->path == FSM_PATH_MIXED);
   L.fe = d_e;
   L.fu = d_u;
   L.fp = d_p;
@@ -731,6 +739,7 @@ struct fsm_batch {
   std::vector<BatchEvent> evs;
   bool dirty = true;
   int use_graph = 1;
+  int built_path = -1;  // context path the descriptors were built for
   DevBuf<char> arena;
   DevBuf<Event2D> d_ev2;
   DevBuf<Event3D> d_ev3;
@@ -761,6 +770,7 @@ const int kLandauChunk = 1024;
 
 int batch_build(fsm_batch *b, hipStream_t s) {
   b->drop_graph();
+  b->built_path = b->ctx->path;
   std::vector<Event2D> e2;
   std::vector<Event3D> e3;
   std::vector<EventLandau> el;
@@ -796,7 +806,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       for (int y = 0; y < d.tiles_y; ++y)
         for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
       b->lds2 = std::max(b->lds2, fsm_stream2d_lds_bytes(d.variant, d.lds_floats4));
-      EventLandau L = landau_event(p, d.T, 0);
+      EventLandau L = landau_event(p, d.T, 0, b->ctx->path == FSM_PATH_MIXED);
       L.out = ev.d_out;
       el.push_back(L);
     } else {
@@ -815,7 +825,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       for (int ie = 0; ie < p.DIM_ETA; ++ie)
         for (int t = 0; t < d.tiles; ++t) s3.push_back(make_int4(i, t, ie, 0));
       b->lds3 = std::max(b->lds3, (size_t)d.lds_floats * sizeof(float));
-      EventLandau L = landau_event(p, d.T, 1);
+      EventLandau L = landau_event(p, d.T, 1, b->ctx->path == FSM_PATH_MIXED);
       L.out = ev.d_out;
       el.push_back(L);
     }
@@ -929,7 +939,7 @@ int fsm_batch_run(fsm_batch *b, void *stream) {
   if (b->evs.empty()) return FSM_OK;
   FSM_TRY(hipSetDevice(b->ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  if (b->dirty) {
+  if (b->dirty || b->built_path != b->ctx->path) {
     int st = batch_build(b, s);
     if (st) return st;
   }
@@ -968,7 +978,7 @@ void batch_set_async_upload(fsm_batch *b, bool on) { b->async_upload = on; }
 
 int batch_launch_now(fsm_batch *b, hipStream_t s) {
   if (b->evs.empty()) return FSM_OK;
-  if (b->dirty) {
+  if (b->dirty || b->built_path != b->ctx->path) {
     int st = batch_build(b, s);
     if (st) return st;
   }

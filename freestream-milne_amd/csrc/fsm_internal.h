@@ -30,11 +30,12 @@
 // Band half-width of the natural-spline derivative operator.  The operator
 // is the inverse of the tridiagonal (1,4,1) spline system times a central
 // difference; its entries decay as (2 - sqrt 3)^|i-m| = 0.268^|i-m|, so
-// entries beyond 24 are below 2e-14 of the diagonal (checked on the host).
-#define FSM_SPLINE_BAND 24
+// entries beyond 16 are below 1e-9 of the largest (checked on the host when
+// the tables are built) -- far below float resolution.
+#define FSM_SPLINE_BAND 16
 #define FSM_SPLINE_W (2 * FSM_SPLINE_BAND + 1)
 #define FSMSP_TX 16
-#define FSMSP_TY 64
+#define FSMSP_TY 32
 
 // One azimuthal sample of the 2-D stencil.  For a fixed phi_j the source
 // point x - DTAU cos(phi_j) is the same shift for every cell
@@ -48,8 +49,9 @@ struct Stencil2D {
   float c, s;       // cos(phi_j), sin(phi_j) as the reference computes them
   float cc, cs, ss; // hypertrig products c*c, c*s, s*s
   float dxj, dyj;   // cos(phi_j)*DTAU, sin(phi_j)*DTAU (float, as the reference)
-  float pad0, pad1, pad2;
+  float pad;
 };
+static_assert(sizeof(Stencil2D) == 80, "Stencil2D is staged in LDS as 5 float4");
 
 // A contiguous range of azimuthal samples whose stencil offsets share a
 // bounding box; the 2-D kernel stages that box of knots in LDS once per range.
@@ -131,6 +133,7 @@ struct EventLandau {
   int eos_type;
   int viscous;
   int fill_outputs;     // OUTPUTFORMAT is 2 or 3
+  int mixed;            // 1: FP32 Landau sweeps with FP64 for ill-conditioned cells
   float tau;
   const float *T;
   long t_plane;

@@ -61,10 +61,21 @@ __device__ __forceinline__ bool mask_ok(int i, int g, float shift, float mn, flo
 #define SP_RX (FSMSP_TX + 2 * FSM_SPLINE_BAND)
 #define SP_RY (FSMSP_TY + 2 * FSM_SPLINE_BAND)
 
-__device__ void spline2d_tile(const Event2D &ev, int tx, int ty, float *zr, float *zxr) {
+// Shared memory of one spline tile: z over the tile plus a band-wide halo,
+// z_x over the tile rows plus the y halo, and the band weights of the tile's
+// x rows and y columns.
+struct SplineLds {
+  float zr[SP_RX * SP_RY];
+  float zxr[FSMSP_TX * SP_RY];
+  float wx[FSMSP_TX * FSM_SPLINE_W];
+  float wy[FSMSP_TY * FSM_SPLINE_W];
+};
+
+__device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
   const int nx = ev.nx, ny = ev.ny;
   const int x0 = tx * FSMSP_TX, y0 = ty * FSMSP_TY;
   const int B = FSM_SPLINE_BAND;
+  __syncthreads();
   for (int k = threadIdx.x; k < SP_RX * SP_RY; k += blockDim.x) {
     int lx = k / SP_RY, ly = k - lx * SP_RY;
     int gx = x0 - B + lx, gy = y0 - B + ly;
@@ -73,7 +84,15 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, float *zr, floa
       float e0 = ingest_e0(ev.e_gev[(long)gx * ny + gy]);
       z = (float)((double)e0 / (2.0 * M_PI));  // src/FreeStream.cpp:318
     }
-    zr[k] = z;
+    L.zr[k] = z;
+  }
+  for (int k = threadIdx.x; k < FSMSP_TX * FSM_SPLINE_W; k += blockDim.x) {
+    const int gx = x0 + k / FSM_SPLINE_W;
+    L.wx[k] = gx < nx ? ev.gx[(long)x0 * FSM_SPLINE_W + k] : 0.0f;
+  }
+  for (int k = threadIdx.x; k < FSMSP_TY * FSM_SPLINE_W; k += blockDim.x) {
+    const int gy = y0 + k / FSM_SPLINE_W;
+    L.wy[k] = gy < ny ? ev.gy[(long)y0 * FSM_SPLINE_W + k] : 0.0f;
   }
   __syncthreads();
   // z_x (in grid units) for the tile rows and the y-halo.
@@ -82,47 +101,44 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, float *zr, floa
     int gx = x0 + lx, gy = y0 - B + ly;
     float acc = 0.0f;
     if (gx < nx && gy >= 0 && gy < ny) {
-      const float *w = ev.gx + (long)gx * FSM_SPLINE_W;
-#pragma unroll 7
-      for (int m = 0; m < FSM_SPLINE_W; ++m) acc = fmaf(w[m], zr[(lx + m) * SP_RY + ly], acc);
+      const float *w = L.wx + lx * FSM_SPLINE_W;
+#pragma unroll
+      for (int m = 0; m < FSM_SPLINE_W; ++m) acc = fmaf(w[m], L.zr[(lx + m) * SP_RY + ly], acc);
     }
-    zxr[k] = acc;
+    L.zxr[k] = acc;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < FSMSP_TX * FSMSP_TY; k += blockDim.x) {
     int lx = k / FSMSP_TY, ly = k - lx * FSMSP_TY;
     int gx = x0 + lx, gy = y0 + ly;
     if (gx < nx && gy < ny) {
-      const float *w = ev.gy + (long)gy * FSM_SPLINE_W;
+      const float *w = L.wy + ly * FSM_SPLINE_W;
       float zy = 0.0f, zxy = 0.0f;
-#pragma unroll 7
+#pragma unroll
       for (int m = 0; m < FSM_SPLINE_W; ++m) {
-        zy = fmaf(w[m], zr[(lx + B) * SP_RY + ly + m], zy);
-        zxy = fmaf(w[m], zxr[lx * SP_RY + ly + m], zxy);
+        zy = fmaf(w[m], L.zr[(lx + B) * SP_RY + ly + m], zy);
+        zxy = fmaf(w[m], L.zxr[lx * SP_RY + ly + m], zxy);
       }
-      float z = zr[(lx + B) * SP_RY + ly + B];
-      float zx = zxr[lx * SP_RY + ly + B];
+      float z = L.zr[(lx + B) * SP_RY + ly + B];
+      float zx = L.zxr[lx * SP_RY + ly + B];
       ev.planes[(long)gx * ny + gy] = make_float4(z, zy, zx, zxy);
     }
   }
-  __syncthreads();
 }
 
 __global__ __launch_bounds__(256) void k_spline2d_one(Event2D ev) {
-  __shared__ float zr[SP_RX * SP_RY];
-  __shared__ float zxr[FSMSP_TX * SP_RY];
+  __shared__ SplineLds L;
   int tiles_y = (ev.ny + FSMSP_TY - 1) / FSMSP_TY;
-  spline2d_tile(ev, blockIdx.x / tiles_y, blockIdx.x % tiles_y, zr, zxr);
+  spline2d_tile(ev, blockIdx.x / tiles_y, blockIdx.x % tiles_y, L);
 }
 
 __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restrict__ evs,
                                                         const int4 *__restrict__ items, int n_items) {
-  __shared__ float zr[SP_RX * SP_RY];
-  __shared__ float zxr[FSMSP_TX * SP_RY];
+  __shared__ SplineLds L;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     int4 item = items[it];
     const Event2D ev = evs[item.x];
-    spline2d_tile(ev, item.y, item.z, zr, zxr);
+    spline2d_tile(ev, item.y, item.z, L);
   }
 }
 
@@ -348,82 +364,109 @@ __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D 
 // boosts maps (1,0,0,0) to u.  Selection rules of the reference
 // (src/LandauMatch.cpp:228-291): e > 0, u timelike, u^tau < 100, else
 // e = 1e-3 and u = (1,0,0,0).
-__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4]) {
-  const double tau = (double)TAU;
-  double A[4][4], M[4][4];
+// Returns 1 when a Landau frame is accepted (e, u written), 0 when the
+// reference's fallback applies (e = 1e-3, u = (1,0,0,0) written).
+// tol2: squared relative off-diagonal residual of convergence; loose2: the
+// residual still accepted after the last sweep.
+template <typename R>
+__device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4]) {
+  const R tau = (R)TAU;
+  R A[4][4], M[4][4];
   A[0][0] = t[0];
   A[0][1] = A[1][0] = t[1];
   A[0][2] = A[2][0] = t[2];
-  A[0][3] = A[3][0] = tau * (double)t[3];
+  A[0][3] = A[3][0] = tau * (R)t[3];
   A[1][1] = t[4];
   A[1][2] = A[2][1] = t[5];
-  A[1][3] = A[3][1] = tau * (double)t[6];
+  A[1][3] = A[3][1] = tau * (R)t[6];
   A[2][2] = t[7];
-  A[2][3] = A[3][2] = tau * (double)t[8];
-  A[3][3] = tau * tau * (double)t[9];
+  A[2][3] = A[3][2] = tau * (R)t[8];
+  A[3][3] = tau * tau * (R)t[9];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) M[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int j = 0; j < 4; ++j) M[i][j] = (i == j) ? (R)1 : (R)0;
   bool ok = true, conv = false;
   for (int sweep = 0; sweep < 60 && ok; ++sweep) {
-    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
-    if (off <= 1.0e-28 * A[0][0] * A[0][0]) {
+    const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
+    if (off <= tol2 * A[0][0] * A[0][0]) {
       conv = true;
       break;
     }
 #pragma unroll
     for (int i = 1; i < 4; ++i) {
-      const double b = A[0][i];
-      if (b == 0.0) continue;
-      const double s = A[0][0] + A[i][i];
-      const double disc = (s - 2.0 * fabs(b)) * (s + 2.0 * fabs(b));
-      if (!(s > 0.0) || !(disc > 0.0)) {
+      const R b = A[0][i];
+      if (b == (R)0) continue;
+      const R s = A[0][0] + A[i][i];
+      const R disc = (s - (R)2 * fabs(b)) * (s + (R)2 * fabs(b));
+      if (!(s > (R)0) || !(disc > (R)0)) {
         ok = false;
         break;
       }
-      const double rq = rsqrt(disc);
-      const double ch2 = s * rq;          // cosh 2theta
-      const double sh2 = -2.0 * b * rq;   // sinh 2theta
-      const double ch = sqrt(0.5 * (ch2 + 1.0));
-      const double sh = sh2 / (2.0 * ch);
+      const R rq = (R)1 / sqrt(disc);
+      const R ch2 = s * rq;          // cosh 2theta
+      const R sh2 = (R)-2 * b * rq;  // sinh 2theta
+      const R ch = sqrt((R)0.5 * (ch2 + (R)1));
+      const R sh = sh2 / ((R)2 * ch);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double a0 = A[0][k], ai = A[i][k];
+        const R a0 = A[0][k], ai = A[i][k];
         A[0][k] = ch * a0 + sh * ai;
         A[i][k] = sh * a0 + ch * ai;
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double a0 = A[k][0], ai = A[k][i];
+        const R a0 = A[k][0], ai = A[k][i];
         A[k][0] = ch * a0 + sh * ai;
         A[k][i] = sh * a0 + ch * ai;
       }
-      A[0][i] = A[i][0] = 0.0;
+      A[0][i] = A[i][0] = (R)0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double m0 = M[k][0], mi = M[k][i];
+        const R m0 = M[k][0], mi = M[k][i];
         M[k][0] = ch * m0 - sh * mi;
         M[k][i] = ch * mi - sh * m0;
       }
     }
   }
   if (ok && !conv) {
-    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
-    conv = off <= 1.0e-12 * A[0][0] * A[0][0];
+    const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
+    conv = off <= loose2 * A[0][0] * A[0][0];
   }
-  const double e = A[0][0];
-  const double u0 = M[0][0];
-  if (ok && conv && e > 0.0 && u0 < 100.0 && isfinite(u0)) {
+  const R e = A[0][0];
+  const R u0 = M[0][0];
+  if (ok && conv && e > (R)0 && u0 < (R)100 && isfinite(u0)) {
     *e_out = (float)e;
     u_out[0] = (float)u0;
     u_out[1] = (float)M[1][0];
     u_out[2] = (float)M[2][0];
     u_out[3] = (float)(M[3][0] / tau);
+    return 1;
+  }
+  *e_out = 1.0e-3f;
+  u_out[0] = 1.0f;
+  u_out[1] = u_out[2] = u_out[3] = 0.0f;
+  return 0;
+}
+
+__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  (void)landau_jacobi<double>(t, TAU, 1.0e-28, 1.0e-12, e_out, u_out);
+}
+
+// Mixed precision: FP32 sweeps, and FP64 only where FP32 cannot decide the
+// result to the parity tolerance -- a failed or unconverged FP32 solve, fast
+// flow (u^tau > 1.5, the frame is conditioned like gamma^2) and nearly null
+// tensors (e < 0.05 T^{tau tau}, dilute edge cells).
+__device__ void landau_mixed(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  float e32, u32[4];
+  const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, &e32, u32);
+  const bool need64 = st != 1 || !(u32[0] <= 1.5f) || !(e32 >= 0.05f * t[0]);
+  if (need64) {
+    landau_fp64(t, TAU, e_out, u_out);
   } else {
-    *e_out = 1.0e-3f;
-    u_out[0] = 1.0f;
-    u_out[1] = u_out[2] = u_out[3] = 0.0f;
+    *e_out = e32;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) u_out[c] = u32[c];
   }
 }
 
@@ -477,7 +520,10 @@ __device__ void landau_cell_out(const EventLandau &ev, long is) {
 #pragma unroll
   for (int c = 0; c < 10; ++c) t[c] = ev.T[c * ev.t_plane + tidx];
   float e, u[4];
-  landau_fp64(t, ev.tau, &e, u);
+  if (ev.mixed)
+    landau_mixed(t, ev.tau, &e, u);
+  else
+    landau_fp64(t, ev.tau, &e, u);
   const float TAU = ev.tau;
   const float p = eos_pressure(e, ev.eos_type);
   float Pi = 0.0f, pi[10];

@@ -66,6 +66,9 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
     // every sample of every real cell of the tile lands in [2, n-3]
     const bool interior = (x0 + G.bx0 >= 2) && (xlast + G.bx1 - 1 <= nx - 3) && (y0 + G.by0 >= 2) &&
                           (ylast + G.by1 - 1 <= ny - 3);
+    const int nj = G.j1 - G.j0;
+    // the group's stencils follow the knot box in LDS (5 float4 each)
+    Stencil2D *sts = reinterpret_cast<Stencil2D *>(lds + LX * LY);
     __syncthreads();
     for (int k = threadIdx.x; k < LX * LY; k += blockDim.x) {
       const int lx = k / LY, ly = k - lx * LY;
@@ -74,31 +77,49 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
       if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = ev.planes[(long)gx * ny + gy];
       lds[k] = v;
     }
+    {
+      const float4 *src = reinterpret_cast<const float4 *>(ev.st + G.j0);
+      float4 *dst = reinterpret_cast<float4 *>(sts);
+      for (int k = threadIdx.x; k < nj * 5; k += blockDim.x) dst[k] = src[k];
+    }
     __syncthreads();
-    for (int j = G.j0 + js; j < G.j1; j += JS) {
-      const Stencil2D S = ev.st[j];
-      const int base = (xr + S.ox - G.bx0) * LY + (yl + S.oy - G.by0);
+    // Samples of a group are sorted by knot offset on the host, so the knot
+    // columns are read again only when the offset changes.
+    int cur_ox = 1 << 30, cur_oy = 1 << 30, base = 0;
+    float4 K0[M + 1], K1[M + 1];
+    for (int jj = js; jj < nj; jj += JS) {
+      const Stencil2D S = sts[jj];
+      const int ox = __builtin_amdgcn_readfirstlane(S.ox), oy = __builtin_amdgcn_readfirstlane(S.oy);
+      if (ox != cur_ox || oy != cur_oy) {
+        cur_ox = ox;
+        cur_oy = oy;
+        base = (xr + ox - G.bx0) * LY + (yl + oy - G.by0);
+#pragma unroll
+        for (int a = 0; a <= M; ++a) {
+          K0[a] = lds[base + a * LY];
+          K1[a] = lds[base + a * LY + 1];
+        }
+      }
       float Zy[M + 1], ZXy[M + 1];
 #pragma unroll
       for (int a = 0; a <= M; ++a) {
-        const float4 k0 = lds[base + a * LY];
-        const float4 k1 = lds[base + a * LY + 1];
+        const float4 k0 = K0[a], k1 = K1[a];
         Zy[a] = fmaf(S.hy[0], k0.x, fmaf(S.hy[1], k0.y, fmaf(S.hy[2], k1.x, S.hy[3] * k1.y)));
         ZXy[a] = fmaf(S.hy[0], k0.z, fmaf(S.hy[1], k0.w, fmaf(S.hy[2], k1.z, S.hy[3] * k1.w)));
       }
       float v[M];
-      bool neg = false;
 #pragma unroll
-      for (int m = 0; m < M; ++m) {
+      for (int m = 0; m < M; ++m)
         v[m] = fmaf(S.hx[0], Zy[m], fmaf(S.hx[1], ZXy[m], fmaf(S.hx[2], Zy[m + 1], S.hx[3] * ZXy[m + 1])));
-        neg |= !(v[m] > 0.0f);
-      }
+      float vmin = v[0];
+#pragma unroll
+      for (int m = 1; m < M; ++m) vmin = fminf(vmin, v[m]);
       // src/FreeStream.cpp:245-246: bilinear where the spline is not > 0
-      if (__ballot(neg)) {
+      if (__ballot(!(vmin > 0.0f))) {
         float Bl[M + 1];
 #pragma unroll
         for (int a = 0; a <= M; ++a) {
-          const float z0 = lds[base + a * LY].x, z1 = lds[base + a * LY + 1].x;
+          const float z0 = K0[a].x, z1 = K1[a].x;
           Bl[a] = fmaf(S.ty, z1 - z0, z0);
         }
 #pragma unroll
@@ -108,10 +129,10 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
         }
       }
       if (!interior) {
-        const bool yok = mask_ok2(iy, iy + S.oy, S.dyj, ev.ymin, ev.dy, ny);
+        const bool yok = mask_ok2(iy, iy + oy, S.dyj, ev.ymin, ev.dy, ny);
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          const bool ok = yok && mask_ok2(ixb + m, ixb + m + S.ox, S.dxj, ev.xmin, ev.dx, nx);
+          const bool ok = yok && mask_ok2(ixb + m, ixb + m + ox, S.dxj, ev.xmin, ev.dx, nx);
           v[m] = ok ? v[m] : 0.0f;
         }
       }
