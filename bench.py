@@ -6,15 +6,16 @@ DX = DY = 0.06 fm, 200 Gaussian hotspots, DIM_PHIP = 200, TAU0 = 0.1,
 DTAU = 1.0 fm/c, conformal EoS, viscous matching -- with the events already
 resident in HBM.  One step = one batch of `--batch` independent events of that
 shape (different seeds) run with one launch per stage, replayed from a HIP
-graph.  `value` is events/s over the timed steps.
+graph, on the mixed-precision path (2-D quadrature on FP16 MFMA with hi+lo
+split, FP32/FP64 Landau).  `value` is events/s over the timed steps.
 
-The line also carries: the plain path (one launch sequence per event, no
-graph) on the same events; single-event latency (p50/p99, device-resident);
-the 131^2 x 101 3-D event; a short serving run (Poisson arrivals through the
-request loop, host inputs, results left in HBM) with p50/p99 latency; the
-dominant kernel's roofline position (HIP events on the launch stream); and the
-reference-algorithm CPU baseline (the oracle's float path, OpenMP on this
-host's cores).
+The line also carries: the same batch on the FP32 path; the plain path (one
+launch sequence per event, no graph) on both paths; single-event latency
+(p50/p99, device-resident); the 131^2 x 101 3-D event; a short serving run
+(Poisson arrivals through the request loop, host inputs, results left in HBM)
+with p50/p99 latency; the stream kernel's roofline position (HIP events on the
+launch stream); and the reference-algorithm CPU baseline (the oracle's float
+path, OpenMP on this host's cores).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -34,6 +35,7 @@ for p in (os.path.join(ROOT, "freestream-milne_amd"), os.path.join(ROOT, "oracle
         sys.path.insert(0, p)
 
 FP32_PEAK_TFLOPS = 157.3
+FP16_MFMA_PEAK_TFLOPS = 2500.0
 FLOP_PER_SAMPLE_2D = 52  # 16 spline taps * 2 + 10 quadrature rows * 2
 FLOP_PER_SAMPLE_3D = 36  # 8 trilinear taps * 2 + 10 * 2
 
@@ -127,7 +129,7 @@ def main():
     stream = torch.cuda.Stream()
     sync = torch.cuda.synchronize
 
-    # ---- C1 batch -----------------------------------------------------------
+    # ---- C1 batch, FP32 and mixed-precision paths ----------------------------
     B = args.batch
     p1, e1 = W.config("C1")
     q1 = fsm_amd.params_finalize(p1, e1)
@@ -135,53 +137,79 @@ def main():
     evs = [e1] + [W.config("C1", seed=100 + k)[1] for k in range(B - 1)]
     d_in = torch.from_numpy(np.stack(evs).astype(np.float32)).to(dev)
     d_out = torch.empty((B, 17, D), dtype=torch.float64, device=dev)
-    batch = fsm_amd.Batch(ctx)
-    for b in range(B):
-        batch.add(q1, d_in[b].data_ptr(), d_out[b].data_ptr())
 
-    def step_batch():
-        batch.run(stream)
-
-    def step_plain():
+    def measure(path):
+        """events/s of the graph-replayed batch and of the plain per-event
+        path, plus per-stage kernel times (HIP events on the launch stream)."""
+        ctx.set_path(path)
+        batch = fsm_amd.Batch(ctx)
         for b in range(B):
-            ctx.run_device(q1, d_in[b].data_ptr(), d_out[b].data_ptr(), stream)
+            batch.add(q1, d_in[b].data_ptr(), d_out[b].data_ptr())
 
-    for _ in range(args.warmup):
-        step_batch()
-        step_plain()
-    el = timed(step_batch, args.steps, sync)
-    value = B * args.steps / el
-    ms_per_step = el * 1e3 / args.steps
-    el_plain = timed(step_plain, max(1, args.steps // 2), sync)
-    plain = B * max(1, args.steps // 2) / el_plain
+        def step_batch():
+            batch.run(stream)
 
-    # per-stage kernel times: the same batch launched directly with HIP events
-    # between stages on the launch stream
-    batch.set_graph(False)
-    ctx.enable_timing(True)
-    ctx.kernel_time_totals()
-    timed(step_batch, args.steps, sync)
-    ktot, ncalls = ctx.kernel_time_totals()
-    ctx.enable_timing(False)
-    batch.set_graph(True)
-    k_ms = ktot[1] / max(ncalls, 1)  # stream + quadrature launch (B events)
+        def step_plain():
+            for b in range(B):
+                ctx.run_device(q1, d_in[b].data_ptr(), d_out[b].data_ptr(), stream)
+
+        for _ in range(args.warmup):
+            step_batch()
+            step_plain()
+        el = timed(step_batch, args.steps, sync)
+        el_plain = timed(step_plain, max(1, args.steps // 2), sync)
+        batch.set_graph(False)
+        ctx.enable_timing(True)
+        ctx.kernel_time_totals()
+        timed(step_batch, args.steps, sync)
+        ktot, ncalls = ctx.kernel_time_totals()
+        ctx.enable_timing(False)
+        batch.close()
+        n = max(ncalls, 1)
+        return {"events_per_s": B * args.steps / el, "ms_per_step": el * 1e3 / args.steps,
+                "plain_events_per_s": B * max(1, args.steps // 2) / el_plain,
+                "stream_launch_ms": ktot[1] / n,
+                "stage_ms_per_event": {"spline": ktot[0] / n / B, "stream": ktot[1] / n / B,
+                                       "landau": ktot[2] / n / B}}
+
+    fp32 = measure(fsm_amd.PATH_FP32)
+    out_fp32 = d_out[:4].cpu().numpy()
+    mixed = measure(fsm_amd.PATH_MIXED)
+    out_mixed = d_out[:4].cpu().numpy()
+    # same events, both paths: largest e difference relative to max e
+    mixed_vs_fp32_e = float((np.abs(out_mixed[:, 0] - out_fp32[:, 0]).max(axis=1) /
+                             np.abs(out_fp32[:, 0]).max(axis=1)).max())
+    value = mixed["events_per_s"]
+    ms_per_step = mixed["ms_per_step"]
+    info = ctx.plan_info(q1)  # context is on the mixed path here
     samples = D * q1.DIM_PHIP * q1.DIM_RAP * B
-    flops = FLOP_PER_SAMPLE_2D * samples
-    achieved = flops / (k_ms * 1e-3) / 1e12
+    k_ms = mixed["stream_launch_ms"]
+    mfma_flops = B * info["tiles"] * (info["tile_w"] * info["tile_h"] // 16) * info["n_offsets"] * 16384
+    achieved = mfma_flops / (k_ms * 1e-3) / 1e12
     roofline = {
-        "kernel": "k_stream2d_batch",
+        "kernel": "k_stream2d_mfma_batch",
         "bound": "mfma",
         "achieved": round(achieved, 3),
-        "peak": FP32_PEAK_TFLOPS,
+        "peak": FP16_MFMA_PEAK_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4),
         "traffic": None,
         "avg_kernel_ms": round(k_ms, 5),
-        "flop_per_launch": flops,
-        "note": "FP32 VALU kernel (no MFMA): 52 algorithmic FLOP per (cell, phi) sample, %d events per launch; "
-                "157.3 TFLOP/s is MI355X's FP32 peak" % B,
+        "flop_per_launch": mfma_flops,
+        "fp32_equivalent_tflops": round(FLOP_PER_SAMPLE_2D * samples / (k_ms * 1e-3) / 1e12, 3),
+        "note": "v_mfma_f32_16x16x32_f16 issue: %d events x %d tiles x %d cell blocks x %d knot offsets x "
+                "16384 FLOP per launch, against MI355X's dense FP16 peak; fp32_equivalent counts the "
+                "reference's 52 FLOP per (cell, phi) sample over the same time" %
+                (B, info["tiles"], info["tile_w"] * info["tile_h"] // 16, info["n_offsets"]),
     }
-    stage_ms = {"spline": ktot[0] / max(ncalls, 1) / B, "stream": k_ms / B, "landau": ktot[2] / max(ncalls, 1) / B}
+    kf = fp32["stream_launch_ms"]
+    roofline_fp32 = {
+        "kernel": "k_stream2d_batch", "bound": "valu",
+        "achieved": round(FLOP_PER_SAMPLE_2D * samples / (kf * 1e-3) / 1e12, 3),
+        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(FLOP_PER_SAMPLE_2D * samples / (kf * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+        "avg_kernel_ms": round(kf, 5),
+    }
 
     # ---- single-event latency (device-resident) ------------------------------
     lat = []
@@ -251,23 +279,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp16 hi+lo MFMA, fp32 accumulate; fp32/fp64 Landau",
         "data": "synthetic",
         "config": {"workload": "2-D lumpy event 261x261, DX=0.06, 200 hotspots, DIM_PHIP=200, DTAU=1.0; "
-                               "%d events per step, device-resident, batched launch replayed from a HIP graph" % B,
-                   "batch": B, "path": "fp32"},
-        "plain_fp32_events_per_s": round(plain, 3),
-        "batched_over_plain": round(value / plain, 3),
+                               "%d events per step, device-resident, batched launch replayed from a HIP graph, "
+                               "mixed-precision path" % B,
+                   "batch": B, "path": "mixed"},
+        "fp32_batched_events_per_s": round(fp32["events_per_s"], 3),
+        "plain_fp32_events_per_s": round(fp32["plain_events_per_s"], 3),
+        "plain_mixed_events_per_s": round(mixed["plain_events_per_s"], 3),
+        "mixed_batched_over_plain_fp32": round(value / fp32["plain_events_per_s"], 3),
+        "mixed_vs_fp32_max_e_diff_rel": mixed_vs_fp32_e,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                       "p99": round(float(np.percentile(lat, 99)), 4)},
-        "stage_ms_per_event": {k: round(v, 5) for k, v in stage_ms.items()},
+                       "p99": round(float(np.percentile(lat, 99)), 4), "path": "mixed"},
+        "stage_ms_per_event": {"mixed": {k: round(v, 5) for k, v in mixed["stage_ms_per_event"].items()},
+                               "fp32": {k: round(v, 5) for k, v in fp32["stage_ms_per_event"].items()}},
+        "plan": info,
         "c2_3d": c2,
         "serving": serve,
         "roofline": roofline,
+        "roofline_fp32": roofline_fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    batch.close()
     ctx.close()
 
 

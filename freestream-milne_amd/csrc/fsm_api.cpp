@@ -124,6 +124,19 @@ struct Plan {
   DevBuf<StencilGroup> groups;
   DevBuf<float> gx, gy;
   DevBuf<float4> planes;
+  // 2-D, MFMA form
+  DevBuf<int4> moffs;
+  DevBuf<uint4> bmat;
+  DevBuf<StencilGroup> mgroups;
+  DevBuf<int> amb;
+  DevBuf<uint4> planes_h;
+  DevBuf<unsigned> sp_flags;
+  DevBuf<float> mscale;
+  // tile shapes: ev2.variant is the one-event (latency) shape; batches use
+  // bvariant[path], the throughput shape, set by the autotuner when enabled
+  int bvariant[2] = {0, 0};
+  bool tuned[2] = {false, false};
+  float tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
   // 3-D
   Event3D ev3{};
   DevBuf<RapSlot> slots;
@@ -134,11 +147,19 @@ struct Plan {
 
 bool same_shape(const fsm_params &a, const fsm_params &b) { return memcmp(&a, &b, sizeof(a)) == 0; }
 
+void set_variant(Event2D &ev, int v) {
+  ev.variant = v;
+  ev.tiles_x = (ev.nx + fsm_stream2d_tile_w(v) - 1) / fsm_stream2d_tile_w(v);
+  ev.tiles_y = (ev.ny + fsm_stream2d_tile_h(v) - 1) / fsm_stream2d_tile_h(v);
+}
+
 }  // namespace
 
 struct fsm_ctx {
   int device = 0;
   int path = FSM_PATH_FP32;
+  int tile_policy = FSM_TILES_AUTO;
+  int autotune = 1;
   int timing = 0;
   int max_lds = 65536;
   int num_cus = 256;
@@ -150,6 +171,7 @@ struct fsm_ctx {
   float last_ms[3] = {0, 0, 0};
   std::vector<std::shared_ptr<Plan>> plans;
   std::mutex plan_mu;  // the plan cache is shared with the server's worker thread
+  std::mutex tune_mu;  // one autotuning run at a time
   // host-API workspace
   DevBuf<float> d_in, d_T;
   DevBuf<double> d_out;
@@ -178,6 +200,126 @@ int check_supported(const fsm_params &p) {
 }
 
 // ---- 2-D plan ---------------------------------------------------------------
+// Floor index of a shifted grid point as the reference computes it in float
+// (src/FreeStream.cpp:163,187,198,204): x = (float)i*D + xmin, x_new = x -
+// shift, floor((x_new - xmin)/D).  volatile keeps each rounding step.
+int ref_floor(int i, float shift, float mn, float h) {
+  volatile float x = (float)i * h;
+  x = x + mn;
+  volatile float xn = x - shift;
+  volatile float d = xn - mn;
+  volatile float q = d / h;
+  return (int)floorf(q);
+}
+
+// True when the reference's floor index is i + o for every i in [0, n).
+bool floor_is_offset(int n, float shift, float mn, float h, int o) {
+  for (int i = 0; i < n; ++i)
+    if (ref_floor(i, shift, mn, h) != i + o) return false;
+  return true;
+}
+
+unsigned short half_bits(double v) {
+  const _Float16 h = (_Float16)v;
+  return __builtin_bit_cast(unsigned short, h);
+}
+
+// Tables of the MFMA form: distinct knot offsets of the samples selected by
+// `use`, their 32x16 weight matrices in MFMA lane order,
+// offset groups whose knot box (for a 32-wide tile) plus weights and offset
+// records fit the LDS budget, and the remaining samples.  `st` is in phi
+// order.  Offset record: (ox, oy, LDS delta in the group's box, 0).
+struct MfmaTables {
+  std::vector<int4> offs;
+  std::vector<uint4> bmat;
+  std::vector<StencilGroup> groups;
+  int lds4 = 0;
+  int rbx0 = 0, rbx1 = 0, rby0 = 0, rby1 = 0;
+};
+
+bool sample_offset_exact(const Stencil2D &S, int nx, int ny, float xmin, float ymin, float dx, float dy) {
+  return floor_is_offset(nx, S.dxj, xmin, dx, S.ox) && floor_is_offset(ny, S.dyj, ymin, dy, S.oy);
+}
+
+void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char> &use, int budget4, MfmaTables &t) {
+  std::vector<std::array<double, 96>> W;  // [corner*4 + comp][moment]
+  for (size_t j = 0; j < st.size(); ++j) {
+    if (!use[j]) continue;
+    const Stencil2D &S = st[j];
+    size_t o = 0;
+    while (o < t.offs.size() && !(t.offs[o].x == S.ox && t.offs[o].y == S.oy)) ++o;
+    if (o == t.offs.size()) {
+      t.offs.push_back(make_int4(S.ox, S.oy, 0, 0));
+      W.emplace_back();
+      W.back().fill(0.0);
+    }
+    const double P[6] = {1.0, S.c, S.s, S.cc, S.cs, S.ss};
+    for (int g = 0; g < 4; ++g) {
+      const int gx = g & 1, gy = g >> 1;
+      const double w[4] = {(double)S.hx[2 * gx] * S.hy[2 * gy], (double)S.hx[2 * gx] * S.hy[2 * gy + 1],
+                           (double)S.hx[2 * gx + 1] * S.hy[2 * gy], (double)S.hx[2 * gx + 1] * S.hy[2 * gy + 1]};
+      for (int c = 0; c < 4; ++c)
+        for (int m = 0; m < 6; ++m) W[o][(g * 4 + c) * 6 + m] += w[c] * P[m];
+    }
+  }
+  const size_t no = t.offs.size();
+  t.bmat.assign(no * 64, make_uint4(0, 0, 0, 0));
+  for (size_t o = 0; o < no; ++o)
+    for (int lane = 0; lane < 64; ++lane) {
+      const int n = lane & 15, g = lane >> 4;
+      unsigned short h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int c = 0; c < 4; ++c) {
+        if (n < 6) {
+          const double w = W[o][(g * 4 + c) * 6 + n];
+          h[c] = h[4 + c] = half_bits(w);
+        } else if (n >= 8 && n < 14) {
+          const double w = W[o][(g * 4 + c) * 6 + n - 8];
+          const _Float16 hi = (_Float16)w;
+          h[c] = half_bits(w - (double)hi);
+        }
+      }
+      t.bmat[o * 64 + lane] = make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
+                                         h[4] | ((unsigned)h[5] << 16), h[6] | ((unsigned)h[7] << 16));
+    }
+  // greedy groups along the phi order of first appearance
+  const int TX = fsm_stream2d_tile_w(0), TY = fsm_stream2d_tile_h(0);
+  t.lds4 = 0;
+  t.rbx0 = t.rby0 = 1 << 30;
+  t.rbx1 = t.rby1 = -(1 << 30);
+  size_t o = 0;
+  while (o < no) {
+    StencilGroup G{};
+    G.j0 = (int)o;
+    G.bx0 = t.offs[o].x;
+    G.bx1 = t.offs[o].x + 1;
+    G.by0 = t.offs[o].y;
+    G.by1 = t.offs[o].y + 1;
+    size_t e = o + 1;
+    for (; e < no; ++e) {
+      const int bx0 = std::min(G.bx0, t.offs[e].x), bx1 = std::max(G.bx1, t.offs[e].x + 1);
+      const int by0 = std::min(G.by0, t.offs[e].y), by1 = std::max(G.by1, t.offs[e].y + 1);
+      if ((TX + bx1 - bx0) * ((TY + by1 - by0 + 15) & ~15) + 65 * (int)(e + 1 - o) > budget4) break;
+      G.bx0 = bx0;
+      G.bx1 = bx1;
+      G.by0 = by0;
+      G.by1 = by1;
+    }
+    G.j1 = (int)e;
+    // knot box columns padded to 16 records (TY rows + halo, rounded up)
+    const int LYP = (TY + G.by1 - G.by0 + 15) & ~15;
+    t.lds4 = std::max(t.lds4, (TX + G.bx1 - G.bx0) * LYP + 65 * (G.j1 - G.j0));
+    // LDS delta of each offset inside the group's knot box
+    for (size_t k = o; k < e; ++k) t.offs[k].z = (t.offs[k].x - G.bx0) * LYP + (t.offs[k].y - G.by0);
+    t.rbx0 = std::min(t.rbx0, G.bx0);
+    t.rbx1 = std::max(t.rbx1, G.bx1);
+    t.rby0 = std::min(t.rby0, G.by0);
+    t.rby1 = std::max(t.rby1, G.by1);
+    t.groups.push_back(G);
+    o = e;
+  }
+  if (no == 0) t.rbx0 = t.rbx1 = t.rby0 = t.rby1 = 0;
+}
+
 int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   const int nx = p.DIM_X, ny = p.DIM_Y, nphi = p.DIM_PHIP;
   std::vector<float> gxw, gyw;
@@ -245,17 +387,35 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     if (lds_max <= budget4 || G >= nphi) break;
   }
   if (fsm_stream2d_lds_bytes(1, lds_max) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
+  const float xmin = (float)((-1.0) * ((double)(float)(nx - 1) / 2.0) * (double)p.DX);
+  const float ymin = (float)((-1.0) * ((double)(float)(ny - 1) / 2.0) * (double)p.DY);
+  // every sample enters W; the rounding-dependent ones (amb below) also get
+  // an edge-mask correction on the VALU
+  std::vector<char> all(nphi, 1);
+  MfmaTables mt;
+  build_mfma_tables(st, all, budget4, mt);
   // Within a group the order of the samples is free (it only changes the
   // summation order): sort by knot offset so equal offsets are adjacent.
   for (const StencilGroup &sg : groups)
     std::stable_sort(st.begin() + sg.j0, st.begin() + sg.j1, [](const Stencil2D &a, const Stencil2D &b) {
       return a.ox != b.ox ? a.ox < b.ox : a.oy < b.oy;
     });
+  std::vector<int> amb;
+  for (int j = 0; j < nphi; ++j)
+    if (!sample_offset_exact(st[j], nx, ny, xmin, ymin, p.DX, p.DY)) amb.push_back(j);
   FSM_TRY(pl.gx.upload(gxw));
   FSM_TRY(pl.gy.upload(gyw));
   FSM_TRY(pl.st.upload(st));
   FSM_TRY(pl.groups.upload(groups));
   FSM_TRY(pl.planes.ensure((size_t)nx * ny));
+  FSM_TRY(pl.moffs.upload(mt.offs));
+  FSM_TRY(pl.bmat.upload(mt.bmat));
+  FSM_TRY(pl.mgroups.upload(mt.groups));
+  FSM_TRY(pl.amb.upload(amb));
+  const int stx = (nx + FSMSP_TX - 1) / FSMSP_TX, sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
+  FSM_TRY(pl.planes_h.ensure((size_t)nx * ny));
+  FSM_TRY(pl.sp_flags.ensure((size_t)stx * sty));
+  FSM_TRY(pl.mscale.ensure(1));
   Event2D &ev = pl.ev2;
   memset(&ev, 0, sizeof(ev));
   ev.nx = nx;
@@ -266,20 +426,19 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.d_phip = (float)((2.0 * M_PI) / (double)(float)nphi);
   ev.tau = p.TAU;
   // One-event launches pick the widest tile shape that still gives at least
-  // two workgroups per CU (the batch path always uses variant 0).
-  ev.variant = 2;
+  // two workgroups per CU; batches start from the widest shape (bvariant).
+  int sv = 2;
   for (int v = 0; v < 3; ++v) {
     int tw = fsm_stream2d_tile_w(v), th = fsm_stream2d_tile_h(v);
     long tiles = (long)((nx + tw - 1) / tw) * ((ny + th - 1) / th);
     if (tiles >= 2L * ctx->num_cus) {
-      ev.variant = v;
+      sv = v;
       break;
     }
   }
-  ev.tiles_x = (nx + fsm_stream2d_tile_w(ev.variant) - 1) / fsm_stream2d_tile_w(ev.variant);
-  ev.tiles_y = (ny + fsm_stream2d_tile_h(ev.variant) - 1) / fsm_stream2d_tile_h(ev.variant);
-  ev.xmin = (float)((-1.0) * ((double)(float)(nx - 1) / 2.0) * (double)p.DX);
-  ev.ymin = (float)((-1.0) * ((double)(float)(ny - 1) / 2.0) * (double)p.DY);
+  set_variant(ev, sv);
+  ev.xmin = xmin;
+  ev.ymin = ymin;
   ev.dx = p.DX;
   ev.dy = p.DY;
   ev.planes = pl.planes.p;
@@ -288,6 +447,23 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.st = pl.st.p;
   ev.groups = pl.groups.p;
   ev.t_plane = p.DIM;
+  ev.mfma = 0;  // set per call from the context's path
+  ev.n_moff = (int)mt.offs.size();
+  ev.n_mgroups = (int)mt.groups.size();
+  ev.n_amb = (int)amb.size();
+  ev.mfma_lds4 = mt.lds4;
+  ev.rbx0 = mt.rbx0;
+  ev.rbx1 = mt.rbx1;
+  ev.rby0 = mt.rby0;
+  ev.rby1 = mt.rby1;
+  ev.sp_tiles_y = sty;
+  ev.planes_h = nullptr;
+  ev.sp_flags = pl.sp_flags.p;
+  ev.mscale = pl.mscale.p;
+  ev.moffs = pl.moffs.p;
+  ev.bmat = pl.bmat.p;
+  ev.mgroups = pl.mgroups.p;
+  ev.amb = pl.amb.p;
   return FSM_OK;
 }
 
@@ -449,6 +625,11 @@ int stage_a(fsm_ctx *ctx, Plan &pl, const fsm_params &p, const float *d_e, float
     Event2D ev = pl.ev2;
     ev.e_gev = d_e;
     ev.T = T_ref;
+    if (ctx->tile_policy == FSM_TILES_BATCH) set_variant(ev, pl.bvariant[ctx->path]);
+    if (ctx->path == FSM_PATH_MIXED) {
+      ev.mfma = 1;
+      ev.planes_h = pl.planes_h.p;
+    }
     tmark(ctx, 0, s);
     FSM_TRY(fsm_launch_spline2d(ev, s));
     tmark(ctx, 1, s);
@@ -497,6 +678,8 @@ void finish_timing(fsm_ctx *ctx) {
     if (hipEventElapsedTime(&ms, set[i], set[i + 1]) == hipSuccess) ctx->last_ms[i] = ms;
   }
 }
+
+int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p);  // batch section
 
 int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double *d_out, float *T_scratch,
                     hipStream_t s) {
@@ -567,6 +750,67 @@ int fsm_set_path(fsm_ctx *ctx, int path) {
   if (!ctx) return FSM_ERR_ARG;
   if (path != FSM_PATH_FP32 && path != FSM_PATH_MIXED) return FSM_ERR_ARG;
   ctx->path = path;
+  return FSM_OK;
+}
+
+int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info) {
+  if (!ctx || !p || !info) return FSM_ERR_ARG;
+  FSM_TRY(hipSetDevice(ctx->device));
+  Plan *pl = nullptr;
+  int st = get_plan(ctx, *p, &pl);
+  if (st) return st;
+  memset(info, 0, sizeof(*info));
+  info->dim3 = pl->is3d ? 1 : 0;
+  if (pl->is3d) {
+    info->tiles = pl->ev3.tiles * pl->ev3.neta;
+    info->n_phi = pl->ev3.nphi;
+    info->lds_bytes = pl->ev3.lds_floats * (int)sizeof(float);
+    return FSM_OK;
+  }
+  Event2D ev = pl->ev2;
+  ev.mfma = ctx->path == FSM_PATH_MIXED;
+  info->variant = ev.variant;
+  info->tile_w = fsm_stream2d_tile_w(ev.variant);
+  info->tile_h = fsm_stream2d_tile_h(ev.variant);
+  info->tiles = ev.tiles_x * ev.tiles_y;
+  info->n_phi = p->DIM_PHIP;
+  info->n_offsets = ev.n_moff;
+  info->n_offset_groups = ev.n_mgroups;
+  info->n_vector_samples = ev.n_amb;
+  info->lds_bytes = (int)fsm_stream2d_event_lds_bytes(ev);
+  const int bv = pl->bvariant[ctx->path];
+  info->batch_variant = bv;
+  info->batch_tiles = ((p->DIM_X + fsm_stream2d_tile_w(bv) - 1) / fsm_stream2d_tile_w(bv)) *
+                      ((p->DIM_Y + fsm_stream2d_tile_h(bv) - 1) / fsm_stream2d_tile_h(bv));
+  info->tuned = pl->tuned[ctx->path] ? 1 : 0;
+  for (int v = 0; v < 3; ++v) info->tune_us[v] = pl->tune_ms[ctx->path][v] * 1000.0f;
+  return FSM_OK;
+}
+
+int fsm_set_tile_policy(fsm_ctx *ctx, int policy) {
+  if (!ctx || (policy != FSM_TILES_AUTO && policy != FSM_TILES_BATCH)) return FSM_ERR_ARG;
+  ctx->tile_policy = policy;
+  return FSM_OK;
+}
+
+int fsm_set_autotune(fsm_ctx *ctx, int on) {
+  if (!ctx) return FSM_ERR_ARG;
+  ctx->autotune = on ? 1 : 0;
+  return FSM_OK;
+}
+
+int fsm_autotune(fsm_ctx *ctx, const fsm_params *p, int *variant) {
+  if (!ctx || !p) return FSM_ERR_ARG;
+  FSM_TRY(hipSetDevice(ctx->device));
+  std::shared_ptr<Plan> pl;
+  int st = get_plan_shared(ctx, *p, &pl);
+  if (st) return st;
+  if (!pl->is3d) {
+    std::lock_guard<std::mutex> lock(ctx->tune_mu);
+    st = autotune2d(ctx, *pl, *p);
+    if (st) return st;
+  }
+  if (variant) *variant = pl->is3d ? 0 : pl->bvariant[ctx->path];
   return FSM_OK;
 }
 
@@ -708,10 +952,134 @@ inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Device workspace of one event in a batch: spline planes + T (2-D), or
 // source planes + eta-major T (3-D).
+size_t spline_flag_count(const fsm_params &p) {
+  return (size_t)((p.DIM_X + FSMSP_TX - 1) / FSMSP_TX) * ((p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY);
+}
+
 size_t event_workspace_bytes(const fsm_params &p) {
   if (p.DIM_ETA == 1)
-    return align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4)) + align256((size_t)10 * p.DIM * sizeof(float));
+    return align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4)) + align256((size_t)10 * p.DIM * sizeof(float)) +
+           align256((size_t)p.DIM_X * p.DIM_Y * sizeof(uint4)) + align256(spline_flag_count(p) * sizeof(unsigned)) +
+           align256(sizeof(float));
   return align256((size_t)p.DIM * sizeof(float)) + align256((size_t)10 * p.DIM * sizeof(float));
+}
+
+// Descriptor of one 2-D event of a batch whose workspace starts at `cur`
+// (layout of event_workspace_bytes): spline planes, T, MFMA planes, spline
+// tile flags, knot scale.  Advances `cur` past the workspace.
+Event2D event2d_in_workspace(const Plan &pl, const fsm_params &p, const float *d_e, int path, int variant,
+                             char *&cur) {
+  Event2D d = pl.ev2;
+  d.e_gev = d_e;
+  set_variant(d, variant);
+  d.planes = (float4 *)cur;
+  cur += align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4));
+  d.T = (float *)cur;
+  cur += align256((size_t)10 * p.DIM * sizeof(float));
+  d.t_plane = p.DIM;
+  uint4 *ph = (uint4 *)cur;
+  cur += align256((size_t)p.DIM_X * p.DIM_Y * sizeof(uint4));
+  d.sp_flags = (unsigned *)cur;
+  cur += align256(spline_flag_count(p) * sizeof(unsigned));
+  d.mscale = (float *)cur;
+  cur += align256(sizeof(float));
+  d.mfma = path == FSM_PATH_MIXED ? 1 : 0;
+  d.planes_h = d.mfma ? ph : nullptr;
+  return d;
+}
+
+// Autotuner of the batch tile shape: times the 2-D stream kernel of this
+// event shape over a synthetic batch (a smooth profile, enough events to
+// give every CU four tiles of the widest shape) for each tile shape on the
+// context's path, and keeps the fastest as the plan's batch shape.
+int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p) {
+  const int path = ctx->path;
+  const int nx = p.DIM_X, ny = p.DIM_Y;
+  const long t0 = (long)((nx + fsm_stream2d_tile_w(0) - 1) / fsm_stream2d_tile_w(0)) *
+                  ((ny + fsm_stream2d_tile_h(0) - 1) / fsm_stream2d_tile_h(0));
+  const int n = (int)std::min<long>(16, std::max<long>(1, (4L * ctx->num_cus + t0 - 1) / t0));
+  std::vector<float> h((size_t)p.DIM);
+  for (int ix = 0; ix < nx; ++ix)
+    for (int iy = 0; iy < ny; ++iy) {
+      const double x = (ix - 0.5 * (nx - 1)) / (0.2 * nx), y = (iy - 0.5 * (ny - 1)) / (0.2 * ny);
+      h[(size_t)ix * ny + iy] = (float)(10.0 * exp(-0.5 * (x * x + y * y)));
+    }
+  DevBuf<float> d_in;
+  FSM_TRY(d_in.upload(h));
+  const size_t wsb = event_workspace_bytes(p);
+  DevBuf<char> ws;
+  FSM_TRY(ws.ensure(wsb * n));
+  DevBuf<Event2D> d_ev;
+  DevBuf<int4> d_items;
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  FSM_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  int st = FSM_OK;
+  float best = 1e30f;
+  int bestv = pl.bvariant[path];
+  if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) st = FSM_ERR_DEVICE;
+  for (int v = 0; v < 3 && st == FSM_OK; ++v) {
+    std::vector<Event2D> evs;
+    std::vector<int4> ks, sp, s2;
+    size_t lds = 0;
+    char *cur = ws.p;
+    const int stx = (nx + FSMSP_TX - 1) / FSMSP_TX, sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
+    for (int i = 0; i < n; ++i) {
+      Event2D d = event2d_in_workspace(pl, p, d_in.p, path, v, cur);
+      if (d.mfma) ks.push_back(make_int4(i, 0, 0, 0));
+      for (int x = 0; x < stx; ++x)
+        for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
+      for (int y = 0; y < d.tiles_y; ++y)
+        for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
+      lds = std::max(lds, fsm_stream2d_event_lds_bytes(d));
+      evs.push_back(d);
+    }
+    std::vector<int4> items(ks);
+    items.insert(items.end(), sp.begin(), sp.end());
+    items.insert(items.end(), s2.begin(), s2.end());
+    if (d_ev.upload(evs) != hipSuccess || d_items.upload(items) != hipSuccess) {
+      st = FSM_ERR_DEVICE;
+      break;
+    }
+    const int4 *it = d_items.p;
+    const int nks = (int)ks.size(), nsp = (int)sp.size(), ns2 = (int)s2.size();
+    hipError_t e = fsm_launch_knot_scale_batch(d_ev.p, it, nks, s);
+    if (e == hipSuccess) e = fsm_launch_spline2d_batch(d_ev.p, it + nks, nsp, s);
+    if (e == hipSuccess) e = fsm_launch_stream2d_batch(d_ev.p, it + nks + nsp, ns2, lds, path == FSM_PATH_MIXED, s);
+    if (e == hipSuccess) e = hipEventRecord(ev0, s);
+    for (int r = 0; r < 3 && e == hipSuccess; ++r)
+      e = fsm_launch_stream2d_batch(d_ev.p, it + nks + nsp, ns2, lds, path == FSM_PATH_MIXED, s);
+    if (e == hipSuccess) e = hipEventRecord(ev1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(ev1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+    if (e != hipSuccess) {
+      st = hip_status(e);
+      break;
+    }
+    pl.tune_ms[path][v] = ms / (3.0f * n);
+    if (ms < best) {
+      best = ms;
+      bestv = v;
+    }
+  }
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  (void)hipStreamSynchronize(s);
+  (void)hipStreamDestroy(s);
+  if (st == FSM_OK) {
+    pl.bvariant[path] = bestv;
+    pl.tuned[path] = true;
+  }
+  return st;
+}
+
+// Tunes a 2-D plan's batch shape for the context's path once, if enabled.
+int maybe_tune(fsm_ctx *ctx, Plan &pl, const fsm_params &p) {
+  if (!ctx->autotune || pl.is3d) return FSM_OK;
+  std::lock_guard<std::mutex> lock(ctx->tune_mu);
+  if (pl.tuned[ctx->path]) return FSM_OK;
+  return autotune2d(ctx, pl, p);
 }
 
 // Host staging for stream-ordered uploads (pinned, grown as needed).
@@ -745,8 +1113,9 @@ struct fsm_batch {
   DevBuf<Event3D> d_ev3;
   DevBuf<EventLandau> d_evl;
   DevBuf<int4> d_items;
-  int off_sp = 0, n_sp = 0, off_s2 = 0, n_s2 = 0, off_src = 0, n_src = 0, off_s3 = 0, n_s3 = 0, off_l = 0,
-      n_l = 0;
+  // work-item ranges per stage: knot scale, spline, 2-D stream, 3-D source, 3-D stream, Landau
+  int off_ks = 0, n_ks = 0, off_sp = 0, n_sp = 0, off_s2 = 0, n_s2 = 0;
+  int off_src = 0, n_src = 0, off_s3 = 0, n_s3 = 0, off_l = 0, n_l = 0;
   size_t lds2 = 0, lds3 = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -774,7 +1143,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   std::vector<Event2D> e2;
   std::vector<Event3D> e3;
   std::vector<EventLandau> el;
-  std::vector<int4> sp, s2, src, s3, ld;
+  std::vector<int4> ks, sp, s2, src, s3, ld;
   size_t bytes = 0;
   for (auto &ev : b->evs)
     if (!ev.ws) bytes += event_workspace_bytes(ev.p);
@@ -790,14 +1159,9 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       arena_cur += event_workspace_bytes(p);
     }
     if (!ev.plan->is3d) {
-      Event2D d = ev.plan->ev2;
-      d.e_gev = ev.d_e;
-      d.planes = (float4 *)cur;
-      cur += align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4));
-      d.T = (float *)cur;
-      cur += align256((size_t)10 * p.DIM * sizeof(float));
-      d.t_plane = p.DIM;
+      Event2D d = event2d_in_workspace(*ev.plan, p, ev.d_e, b->ctx->path, ev.plan->bvariant[b->ctx->path], cur);
       const int i = (int)e2.size();
+      if (d.mfma) ks.push_back(make_int4(i, 0, 0, 0));
       e2.push_back(d);
       const int stx = (p.DIM_X + FSMSP_TX - 1) / FSMSP_TX, sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
       for (int x = 0; x < stx; ++x)
@@ -805,7 +1169,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       // each event keeps its plan's tile shape (identical bits alone and batched)
       for (int y = 0; y < d.tiles_y; ++y)
         for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
-      b->lds2 = std::max(b->lds2, fsm_stream2d_lds_bytes(d.variant, d.lds_floats4));
+      b->lds2 = std::max(b->lds2, fsm_stream2d_event_lds_bytes(d));
       EventLandau L = landau_event(p, d.T, 0, b->ctx->path == FSM_PATH_MIXED);
       L.out = ev.d_out;
       el.push_back(L);
@@ -837,6 +1201,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     *n = (int)v.size();
     items.insert(items.end(), v.begin(), v.end());
   };
+  put(ks, &b->off_ks, &b->n_ks);
   put(sp, &b->off_sp, &b->n_sp);
   put(s2, &b->off_s2, &b->n_s2);
   put(src, &b->off_src, &b->n_src);
@@ -874,10 +1239,12 @@ int batch_launch(fsm_batch *b, hipStream_t s, bool timed) {
   fsm_ctx *ctx = b->ctx;
   const int4 *it = b->d_items.p;
   if (timed) tmark(ctx, 0, s);
+  FSM_TRY(fsm_launch_knot_scale_batch(b->d_ev2.p, it + b->off_ks, b->n_ks, s));
   FSM_TRY(fsm_launch_spline2d_batch(b->d_ev2.p, it + b->off_sp, b->n_sp, s));
   FSM_TRY(fsm_launch_src3d_batch(b->d_ev3.p, it + b->off_src, b->n_src, s));
   if (timed) tmark(ctx, 1, s);
-  FSM_TRY(fsm_launch_stream2d_batch(b->d_ev2.p, it + b->off_s2, b->n_s2, b->lds2, s));
+  const int mfma = b->built_path == FSM_PATH_MIXED;
+  FSM_TRY(fsm_launch_stream2d_batch(b->d_ev2.p, it + b->off_s2, b->n_s2, b->lds2, mfma, s));
   FSM_TRY(fsm_launch_stream3d_batch(b->d_ev3.p, it + b->off_s3, b->n_s3, b->lds3, s));
   if (timed) tmark(ctx, 2, s);
   FSM_TRY(fsm_launch_landau_batch(b->d_evl.p, it + b->off_l, b->n_l, s));
@@ -909,6 +1276,8 @@ int fsm_batch_add(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, doubl
   FSM_TRY(hipSetDevice(b->ctx->device));
   BatchEvent ev;
   int st = get_plan_shared(b->ctx, *p, &ev.plan);
+  if (st) return st;
+  st = maybe_tune(b->ctx, *ev.plan, *p);
   if (st) return st;
   ev.p = *p;
   ev.d_e = d_e_GeV;
@@ -989,7 +1358,9 @@ int ctx_device(const fsm_ctx *ctx) { return ctx->device; }
 
 int plan_check(fsm_ctx *ctx, const fsm_params *p) {
   std::shared_ptr<Plan> sp;
-  return get_plan_shared(ctx, *p, &sp);
+  int st = get_plan_shared(ctx, *p, &sp);
+  if (st) return st;
+  return maybe_tune(ctx, *sp, *p);
 }
 
 }  // namespace fsm_host

@@ -18,6 +18,17 @@
 
 #define FSM_HBARC_F 0.197326938f
 
+// XCD-aware work order.  An MI355X dispatches workgroups to its 8 XCDs round
+// robin (workgroup b on XCD b % 8), and each XCD has its own L2.  Sending a
+// contiguous range of work items to each XCD keeps neighbouring tiles, which
+// read overlapping knot boxes, in one L2.  A bijection of [0, n) when every
+// index b of a grid-stride loop has b % 8 equal to its workgroup's XCD (the
+// grid covers n, or its size is a multiple of 8).
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int q = n >> 3, rem = n & 7, x = b & 7, j = b >> 3;
+  return x * q + (x < rem ? x : rem) + j;
+}
+
 // ---------------------------------------------------------------- 2-D ----
 // Cells per lane along x in the 2-D stream kernel; one wave covers 64
 // consecutive iy.  Tile = (4 waves * FSM2D_M) x 64 cells.
@@ -81,7 +92,40 @@ struct Event2D {
   const StencilGroup *groups;
   float *T;             // 10 planes, reference layout
   long t_plane;         // plane stride of T (>= DIM)
+  // ---- mixed-precision quadrature (MFMA tiles) ----
+  int mfma;             // 1: MFMA tile kernel; spline kernel also writes planes_h and sp_flags
+  int n_moff;           // knot offsets carrying at least one MFMA sample
+  int n_mgroups;        // offset groups (StencilGroup with j0/j1 = offset range)
+  int n_amb;            // samples whose floor index is rounding-dependent (edge-mask correction)
+  int mfma_lds4;        // 16-byte LDS elements of the largest offset group (box + weights + records)
+  int rbx0, rbx1, rby0, rby1;  // knot box of all offsets (as StencilGroup.bx0..by1)
+  int sp_tiles_y;       // spline tiles along y (flag index = sx * sp_tiles_y + sy)
+  uint4 *planes_h;      // per knot 8 halves: scale*{z, zy, zx, zxy} hi, then lo
+  unsigned *sp_flags;   // per spline tile: nonzero if a knot is unsafe for the MFMA form
+  float *mscale;        // per event: power-of-two knot scale of planes_h (k_knot_scale)
+  const int4 *moffs;    // [n_moff] (ox, oy, LDS delta of the offset in its group's knot box, -)
+  const uint4 *bmat;    // [n_moff][64] MFMA B operand per lane (8 halves)
+  const StencilGroup *mgroups;
+  const int *amb;       // [n_amb] indices into st
 };
+
+// The MFMA form of the 2-D quadrature.  For all samples j sharing a knot
+// offset o the interpolation and the angular moments are linear in the 16
+// knot values of the patch, so
+//   T_m(cell) = sum_o sum_{corner,comp} K[cell+o+corner][comp] * W_o[corner,comp][m],
+//   W_o = sum_{j in o} hermite_j[corner,comp] * P_j[m],
+// one 16x16x32 FP16 MFMA per 16 cells and offset, with the knots and the
+// weights split into hi + lo halves (K_h W_h + K_l W_h + K_h W_l).  Knots are
+// multiplied by a per-event power of two before the split, chosen from the
+// event's largest |z| so that the largest knot value is near
+// FSM_MFMA_ZTARGET (the fp16 subnormal floor then sits ~2^-36 below it; the
+// scale is undone in FP32 at the end).  A spline tile is flagged when a
+// scaled component exceeds FSM_MFMA_MAXABS, or when a patch could go <= 0
+// (Bernstein control points, where the reference falls back to bilinear),
+// and MFMA tiles that read a flagged knot run the FP32 sample loop instead.
+#define FSM_MFMA_ZTARGET 4096.0f
+#define FSM_MFMA_MAXABS 30000.0f
+#define FSM_MFMA_POS_EPS 1.0e-5f
 
 // ---------------------------------------------------------------- 3-D ----
 #define FSM3D_THREADS 256

@@ -21,6 +21,8 @@ hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d);
 int fsm_stream2d_tile_w(int variant);
 int fsm_stream2d_tile_h(int variant);
 size_t fsm_stream2d_lds_bytes(int variant, int box_floats4);
+// Dynamic LDS of one event's stream tiles (FP32 or MFMA form, per ev.mfma).
+size_t fsm_stream2d_event_lds_bytes(const Event2D &ev);
 hipError_t fsm_stream2d_set_lds_limit(int bytes);
 
 
@@ -28,7 +30,10 @@ hipError_t fsm_stream2d_set_lds_limit(int bytes);
 // Batched launchers: `items` are int4 work items in device memory,
 // `evs` device arrays of event descriptors (see fsm_api.cpp, fsm_batch).
 hipError_t fsm_launch_spline2d_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s);
-hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s);
+// One item (event, -, -) per 2-D event of the MFMA form: its knot scale.
+hipError_t fsm_launch_knot_scale_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s);
+hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int n, size_t lds_bytes, int mfma,
+                                     hipStream_t s);
 hipError_t fsm_launch_src3d_batch(const Event3D *evs, const int4 *items, int n, hipStream_t s);
 hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s);
 hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s);

@@ -69,13 +69,35 @@ struct SplineLds {
   float zxr[FSMSP_TX * SP_RY];
   float wx[FSMSP_TX * FSM_SPLINE_W];
   float wy[FSMSP_TY * FSM_SPLINE_W];
+  float red[4];
 };
+
+// Halves of the MFMA knot format: h = fp16(x), l = fp16(x - h).
+__device__ __forceinline__ void split_half(float x, unsigned short &h, unsigned short &l) {
+  const _Float16 hv = (_Float16)x;
+  const _Float16 lv = (_Float16)(x - (float)hv);
+  h = __builtin_bit_cast(unsigned short, hv);
+  l = __builtin_bit_cast(unsigned short, lv);
+}
+
+// Smallest Bernstein control point over the four bicubic patches a knot is a
+// corner of.  A patch with all 16 control points > 0 is > 0 everywhere
+// (convex hull), so the reference never falls back to bilinear inside it.
+__device__ __forceinline__ float knot_cp_min(float z, float zy, float zx, float zxy) {
+  const float a = zx * (1.0f / 3.0f), b = zy * (1.0f / 3.0f), c = zxy * (1.0f / 9.0f);
+  float m = fminf(z, fminf(z + a, z - a));
+  m = fminf(m, fminf(z + b, z - b));
+  m = fminf(m, fminf(z + a + b + c, z - a - b + c));
+  m = fminf(m, fminf(z + a - b - c, z - a + b - c));
+  return m;
+}
 
 __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
   const int nx = ev.nx, ny = ev.ny;
   const int x0 = tx * FSMSP_TX, y0 = ty * FSMSP_TY;
   const int B = FSM_SPLINE_BAND;
   __syncthreads();
+  float zabs = 0.0f;
   for (int k = threadIdx.x; k < SP_RX * SP_RY; k += blockDim.x) {
     int lx = k / SP_RY, ly = k - lx * SP_RY;
     int gx = x0 - B + lx, gy = y0 - B + ly;
@@ -85,6 +107,13 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
       z = (float)((double)e0 / (2.0 * M_PI));  // src/FreeStream.cpp:318
     }
     L.zr[k] = z;
+    zabs = fmaxf(zabs, fabsf(z));
+  }
+  if (ev.planes_h) {
+    // largest |z| within the band of the tile: the scale of the rounding
+    // noise in the knot slopes, below which a negative control point is noise
+    for (int o = 32; o > 0; o >>= 1) zabs = fmaxf(zabs, __shfl_xor(zabs, o));
+    if ((threadIdx.x & 63) == 0) L.red[threadIdx.x >> 6] = zabs;
   }
   for (int k = threadIdx.x; k < FSMSP_TX * FSM_SPLINE_W; k += blockDim.x) {
     const int gx = x0 + k / FSM_SPLINE_W;
@@ -108,6 +137,7 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
     L.zxr[k] = acc;
   }
   __syncthreads();
+  int bad = 0;
   for (int k = threadIdx.x; k < FSMSP_TX * FSMSP_TY; k += blockDim.x) {
     int lx = k / FSMSP_TY, ly = k - lx * FSMSP_TY;
     int gx = x0 + lx, gy = y0 + ly;
@@ -122,21 +152,79 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
       float z = L.zr[(lx + B) * SP_RY + ly + B];
       float zx = L.zxr[lx * SP_RY + ly + B];
       ev.planes[(long)gx * ny + gy] = make_float4(z, zy, zx, zxy);
+      if (ev.planes_h) {
+        const float sc = *ev.mscale;
+        const float tol = -FSM_MFMA_POS_EPS * fmaxf(fmaxf(L.red[0], L.red[1]), fmaxf(L.red[2], L.red[3]));
+        const float big = fmaxf(fmaxf(fabsf(z), fabsf(zy)), fmaxf(fabsf(zx), fabsf(zxy))) * sc;
+        bad |= !(knot_cp_min(z, zy, zx, zxy) >= tol) || !(big <= FSM_MFMA_MAXABS);
+        unsigned short h[4], l[4];
+        split_half(z * sc, h[0], l[0]);
+        split_half(zy * sc, h[1], l[1]);
+        split_half(zx * sc, h[2], l[2]);
+        split_half(zxy * sc, h[3], l[3]);
+        ev.planes_h[(long)gx * ny + gy] =
+            make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16), l[0] | ((unsigned)l[1] << 16),
+                       l[2] | ((unsigned)l[3] << 16));
+      }
     }
+  }
+  if (ev.planes_h) {
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) ev.sp_flags[tx * ev.sp_tiles_y + ty] = bad ? 1u : 0u;
+  }
+}
+
+// Knot scale of the MFMA form: 2^k <= FSM_MFMA_ZTARGET / max|z| over the
+// event (z = e0 / 2 pi, as the spline kernel computes it).  One workgroup
+// per event, no pre-cleared memory (the kernel may be captured in a graph).
+__device__ void knot_scale_event(const Event2D &ev, float *red) {
+  const long n = (long)ev.nx * ev.ny;
+  float m = 0.0f;
+  for (long k = threadIdx.x; k < n; k += blockDim.x)
+    m = fmaxf(m, fabsf((float)((double)ingest_e0(ev.e_gev[k]) / (2.0 * M_PI))));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
+    float s = 1.0f;
+    if (m > 0.0f && m < INFINITY) {
+      int ex = 0;
+      frexpf(FSM_MFMA_ZTARGET / m, &ex);
+      s = ldexpf(1.0f, min(max(ex - 1, -60), 60));
+    }
+    *ev.mscale = s;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_knot_scale_one(Event2D ev) {
+  __shared__ float red[16];
+  knot_scale_event(ev, red);
+}
+
+__global__ __launch_bounds__(1024) void k_knot_scale_batch(const Event2D *__restrict__ evs,
+                                                           const int4 *__restrict__ items, int n_items) {
+  __shared__ float red[16];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const Event2D ev = evs[items[it].x];
+    if (ev.planes_h) knot_scale_event(ev, red);
+    __syncthreads();
   }
 }
 
 __global__ __launch_bounds__(256) void k_spline2d_one(Event2D ev) {
   __shared__ SplineLds L;
   int tiles_y = (ev.ny + FSMSP_TY - 1) / FSMSP_TY;
-  spline2d_tile(ev, blockIdx.x / tiles_y, blockIdx.x % tiles_y, L);
+  const int t = xcd_order(blockIdx.x, gridDim.x);
+  spline2d_tile(ev, t / tiles_y, t % tiles_y, L);
 }
 
 __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restrict__ evs,
                                                         const int4 *__restrict__ items, int n_items) {
   __shared__ SplineLds L;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];
+    int4 item = items[xcd_order(it, n_items)];
     const Event2D ev = evs[item.x];
     spline2d_tile(ev, item.y, item.z, L);
   }
@@ -178,7 +266,7 @@ __global__ __launch_bounds__(256) void k_src3d_batch(const Event3D *__restrict__
                                                      const int4 *__restrict__ items, int n_items) {
   __shared__ float tile[64][65];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];
+    int4 item = items[xcd_order(it, n_items)];
     const Event3D ev = evs[item.x];
     src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, item.y, item.z, tile);
   }
@@ -335,7 +423,8 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
 
 __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_one(Event3D ev) {
   extern __shared__ float qlds[];
-  stream3d_tile(ev, blockIdx.x % ev.tiles, blockIdx.x / ev.tiles, qlds);
+  const int t = xcd_order(blockIdx.x, gridDim.x);
+  stream3d_tile(ev, t % ev.tiles, t / ev.tiles, qlds);
 }
 
 __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D *__restrict__ evs,
@@ -343,7 +432,7 @@ __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D 
                                                                   int n_items) {
   extern __shared__ float qlds[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];
+    int4 item = items[xcd_order(it, n_items)];
     const Event3D ev = evs[item.x];
     stream3d_tile(ev, item.y, item.z, qlds);
   }
@@ -455,12 +544,19 @@ __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_
 
 // Mixed precision: FP32 sweeps, and FP64 only where FP32 cannot decide the
 // result to the parity tolerance -- a failed or unconverged FP32 solve, fast
-// flow (u^tau > 1.5, the frame is conditioned like gamma^2) and nearly null
-// tensors (e < 0.05 T^{tau tau}, dilute edge cells).
+// flow (u^tau > 1.5, the frame is conditioned like gamma^2) and cells whose
+// e is a small difference of large terms (kappa > 16, nearly null tensors in
+// dilute edge cells), where FP32 rounding in T is amplified by kappa.
 __device__ void landau_mixed(const float t[10], float TAU, float *e_out, float u_out[4]) {
   float e32, u32[4];
   const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, &e32, u32);
-  const bool need64 = st != 1 || !(u32[0] <= 1.5f) || !(e32 >= 0.05f * t[0]);
+  // First-order sensitivity of e = T_{mu nu} u^mu u^nu to a relative change
+  // of every component: kappa = sum |u_mu| |T^{mu nu}| |u_nu| / e (>= u0^2 T^tt / e).
+  const float a0 = fabsf(u32[0]), a1 = fabsf(u32[1]), a2 = fabsf(u32[2]), a3 = TAU * TAU * fabsf(u32[3]);
+  const float S = a0 * a0 * fabsf(t[0]) + a1 * a1 * fabsf(t[4]) + a2 * a2 * fabsf(t[7]) + a3 * a3 * fabsf(t[9]) +
+                  2.0f * (a0 * (a1 * fabsf(t[1]) + a2 * fabsf(t[2]) + a3 * fabsf(t[3])) +
+                          a1 * (a2 * fabsf(t[5]) + a3 * fabsf(t[6])) + a2 * a3 * fabsf(t[8]));
+  const bool need64 = st != 1 || !(u32[0] <= 1.5f) || !(S <= 16.0f * e32);
   if (need64) {
     landau_fp64(t, TAU, e_out, u_out);
   } else {
@@ -588,7 +684,7 @@ __global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
 __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restrict__ evs,
                                                       const int4 *__restrict__ items, int n_items) {
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];
+    int4 item = items[xcd_order(it, n_items)];
     const EventLandau ev = evs[item.x];
     for (int c = threadIdx.x; c < item.z; c += blockDim.x) landau_cell_out(ev, (long)item.y + c);
   }
@@ -600,6 +696,11 @@ __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restr
 // Launchers (host)
 // ======================================================================
 hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s) {
+  if (ev.planes_h) {
+    hipLaunchKernelGGL(k_knot_scale_one, dim3(1), dim3(1024), 0, s, ev);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   int tiles = ((ev.nx + FSMSP_TX - 1) / FSMSP_TX) * ((ev.ny + FSMSP_TY - 1) / FSMSP_TY);
   hipLaunchKernelGGL(k_spline2d_one, dim3(tiles), dim3(256), 0, s, ev);
   return hipGetLastError();
@@ -643,6 +744,12 @@ hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d) {
 
 // ---- batched launchers: one launch per stage for every event of a batch ----
 static inline int batch_grid(int n_items) { return std::max(1, std::min(n_items, 16384)); }
+
+hipError_t fsm_launch_knot_scale_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_knot_scale_batch, dim3(batch_grid(n)), dim3(1024), 0, s, evs, items, n);
+  return hipGetLastError();
+}
 
 hipError_t fsm_launch_spline2d_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;

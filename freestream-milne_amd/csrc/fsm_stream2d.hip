@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "fsm_internal.h"
 #include "fsm_kernels.h"
 
@@ -37,6 +39,56 @@ __device__ __forceinline__ bool ref_mask_ok2(int i, float shift, float mn, float
 __device__ __forceinline__ bool mask_ok2(int i, int g, float shift, float mn, float h, int n) {
   if ((unsigned)g <= 1u || (unsigned)(g - (n - 2)) <= 1u) return ref_mask_ok2(i, shift, mn, h, n);
   return (unsigned)(g - 1) <= (unsigned)(n - 3);
+}
+
+// Copies the knot box [bx, bx+LX) x [by, by+LY) of an (nx, ny) x-major plane
+// into LDS (zero outside the grid), column pitch `pitch` >= LY elements.  Each
+// thread keeps U loads in flight and walks the box with an incremental (row,
+// column) instead of a division.
+template <int U, typename T>
+__device__ __forceinline__ void stage_box(T *dst, const T *__restrict__ src, int bx, int by, int LX, int LY, int nx,
+                                          int ny, int pitch) {
+  const int n = LX * LY, step = blockDim.x;
+  const int dlx = step / LY, dly = step - dlx * LY;
+  int lx = threadIdx.x / LY, ly = threadIdx.x - lx * LY;
+  for (int k0 = threadIdx.x; k0 < n; k0 += U * step) {
+    T v[U];
+    int di[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kx = bx + lx, ky = by + ly;
+      di[u] = k0 + u * step < n ? lx * pitch + ly : -1;
+      v[u] = T{};
+      if (di[u] >= 0 && (unsigned)kx < (unsigned)nx && (unsigned)ky < (unsigned)ny) v[u] = src[(long)kx * ny + ky];
+      lx += dlx;
+      ly += dly;
+      if (ly >= LY) {
+        ly -= LY;
+        ++lx;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (di[u] >= 0) dst[di[u]] = v[u];
+  }
+}
+
+// Copies n 16-byte elements with U loads in flight per thread.
+template <int U, typename T>
+__device__ __forceinline__ void stage_copy(T *dst, const T *__restrict__ src, int n) {
+  for (int k0 = threadIdx.x; k0 < n; k0 += U * blockDim.x) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * blockDim.x;
+      if (k < n) v[u] = src[k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * blockDim.x;
+      if (k < n) dst[k] = v[u];
+    }
+  }
 }
 
 template <int M, int JS>
@@ -70,18 +122,8 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
     // the group's stencils follow the knot box in LDS (5 float4 each)
     Stencil2D *sts = reinterpret_cast<Stencil2D *>(lds + LX * LY);
     __syncthreads();
-    for (int k = threadIdx.x; k < LX * LY; k += blockDim.x) {
-      const int lx = k / LY, ly = k - lx * LY;
-      const int gx = x0 + G.bx0 + lx, gy = y0 + G.by0 + ly;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = ev.planes[(long)gx * ny + gy];
-      lds[k] = v;
-    }
-    {
-      const float4 *src = reinterpret_cast<const float4 *>(ev.st + G.j0);
-      float4 *dst = reinterpret_cast<float4 *>(sts);
-      for (int k = threadIdx.x; k < nj * 5; k += blockDim.x) dst[k] = src[k];
-    }
+    stage_box<4>(lds, ev.planes, x0 + G.bx0, y0 + G.by0, LX, LY, nx, ny, LY);
+    stage_copy<4>(reinterpret_cast<float4 *>(sts), reinterpret_cast<const float4 *>(ev.st + G.j0), nj * 5);
     __syncthreads();
     // Samples of a group are sorted by knot offset on the host, so the knot
     // columns are read again only when the offset changes.
@@ -194,10 +236,272 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision tile: the knot-offset form of the quadrature on FP16 MFMA
+// (the algebra is described above FSM_MFMA_ZTARGET in fsm_internal.h).
+//
+// v_mfma_f32_16x16x32_f16 operand layout (wave64): lane l holds A[l%16][8*(l/16)+0..7],
+// B[8*(l/16)+0..7][l%16], and D[4*(l/16)+0..3][l%16].  Rows of A are 16 cells along y,
+// the K index is 8*corner + (hi z, zy, zx, zxy, lo z, zy, zx, zxy), so lane
+// (corner, cell) reads its knot's 16-byte record from LDS as one load; columns
+// 0-5 of B carry W_h for both halves and columns 8-13 carry W_l for the hi half,
+// so D[:, m] + D[:, m+8] = K_h W_h + K_l W_h + K_h W_l for moment m
+// (1, c, s, c^2, cs, s^2).
+//
+// Wave w owns tile columns [w*XW, (w+1)*XW) x 32 rows (two 16-cell blocks).
+// Samples whose shifted floor index is not the same for every cell (the
+// stencil offset is rounding-dependent, e.g. sin(0) = 0) are in W too; for
+// cells where that makes the edge mask rounding-dependent a VALU pass applies
+// the reference's exact decision (sample_correction).
+typedef _Float16 fsm_half8 __attribute__((ext_vector_type(8)));
+typedef float fsm_float4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 plane_or_zero(const Event2D &ev, int gx, int gy) {
+  if ((unsigned)gx < (unsigned)ev.nx && (unsigned)gy < (unsigned)ev.ny) return ev.planes[(long)gx * ev.ny + gy];
+  return make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Edge correction of one rounding-dependent sample of one cell.  The MFMA
+// form adds the bicubic value at the stencil offset whenever the offset's
+// floor index ix + ox is in [1, n-2].  For a sample whose shift is within
+// rounding of an integer the reference's floor can be ix + ox -+ 1; away from
+// the mask boundary that does not change the decision (and the value is
+// continuous across the knot line), but for a floor index in {0, 1, n-2, n-1}
+// the reference's own float expression decides (mask_ok2).  Returns the exact
+// sample (bilinear where the bicubic is not > 0, src/FreeStream.cpp:245-246)
+// minus what the MFMA form added.
+__device__ __forceinline__ float sample_correction(const Event2D &ev, const Stencil2D &S, int ix, int iy) {
+  const int gx = ix + S.ox, gy = iy + S.oy;
+  const bool bx = (unsigned)gx <= 1u || (unsigned)(gx - (ev.nx - 2)) <= 1u;
+  const bool by = (unsigned)gy <= 1u || (unsigned)(gy - (ev.ny - 2)) <= 1u;
+  if (!bx && !by) return 0.0f;
+  const bool idx = (unsigned)(gx - 1) <= (unsigned)(ev.nx - 3) && (unsigned)(gy - 1) <= (unsigned)(ev.ny - 3);
+  const bool ok = mask_ok2(iy, gy, S.dyj, ev.ymin, ev.dy, ev.ny) && mask_ok2(ix, gx, S.dxj, ev.xmin, ev.dx, ev.nx);
+  if (!ok && !idx) return 0.0f;
+  const float4 k00 = plane_or_zero(ev, gx, gy), k01 = plane_or_zero(ev, gx, gy + 1);
+  const float4 k10 = plane_or_zero(ev, gx + 1, gy), k11 = plane_or_zero(ev, gx + 1, gy + 1);
+  const float zy0 = fmaf(S.hy[0], k00.x, fmaf(S.hy[1], k00.y, fmaf(S.hy[2], k01.x, S.hy[3] * k01.y)));
+  const float zxy0 = fmaf(S.hy[0], k00.z, fmaf(S.hy[1], k00.w, fmaf(S.hy[2], k01.z, S.hy[3] * k01.w)));
+  const float zy1 = fmaf(S.hy[0], k10.x, fmaf(S.hy[1], k10.y, fmaf(S.hy[2], k11.x, S.hy[3] * k11.y)));
+  const float zxy1 = fmaf(S.hy[0], k10.z, fmaf(S.hy[1], k10.w, fmaf(S.hy[2], k11.z, S.hy[3] * k11.w)));
+  const float v = fmaf(S.hx[0], zy0, fmaf(S.hx[1], zxy0, fmaf(S.hx[2], zy1, S.hx[3] * zxy1)));
+  float exact = v;
+  if (!(v > 0.0f)) {
+    const float b0 = fmaf(S.ty, k01.x - k00.x, k00.x), b1 = fmaf(S.ty, k11.x - k10.x, k10.x);
+    exact = fmaf(S.tx, b1 - b0, b0);
+  }
+  return (ok ? exact : 0.0f) - (idx ? v : 0.0f);
+}
+
+// The MFMA loop over the knot offsets of one LDS stage, for the wave's XW
+// cell columns x 2 row blocks.  Adjacent cell columns share a knot column:
+// register R[k] holds knot column 2k (lanes with corner-x 0) and 2k+1 (corner
+// x 1), which is the operand of cell column 2k with weights B; cell column
+// 2k+1 takes knot column 2k+1 from R[k] and 2k+2 from R[k+1] (a lane select)
+// with weights B' (corner-x roles swapped: B'[lane] = B[lane ^ 16], a lane
+// swizzle).  So a wave loads XW/2 + 1 knot-column pairs per row block instead
+// of XW.
+// MASKED: src/FreeStream.cpp:213 mask, exact here (the floor index of an MFMA
+// sample is ix + ox for every cell): per offset a cell column is in or out as
+// a whole, a 16-row block is all in, all out, or split (per-lane select);
+// blocks with no sample in range issue no MFMA.
+template <int XW, bool MASKED>
+__device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
+                                             const int4 *orec, int no, int lane_base, int LYP, int gx, int r,
+                                             int xw0, int y0, int nx, int ny) {
+  const int lane = threadIdx.x & 63;
+  constexpr int NR = XW / 2 + 1;
+  for (int o = 0; o < no; ++o) {
+    const int4 rec = orec[o];
+    const int ox = __builtin_amdgcn_readfirstlane(rec.x), oy = __builtin_amdgcn_readfirstlane(rec.y);
+    const int delta = __builtin_amdgcn_readfirstlane(rec.z);
+    const uint4 bw = bl[o * 64 + lane];
+    const fsm_half8 b0 = __builtin_bit_cast(fsm_half8, bw);
+    // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 -> lane ^ 16 within 32 lanes
+    const uint4 bs = make_uint4(__builtin_amdgcn_ds_swizzle((int)bw.x, 0x401F), __builtin_amdgcn_ds_swizzle((int)bw.y, 0x401F),
+                                __builtin_amdgcn_ds_swizzle((int)bw.z, 0x401F), __builtin_amdgcn_ds_swizzle((int)bw.w, 0x401F));
+    const fsm_half8 b1 = __builtin_bit_cast(fsm_half8, bs);
+    uint4 R[NR][2];
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+#pragma unroll
+      for (int yb = 0; yb < 2; ++yb) {
+        // the last pair is used only by corner-x 1 lanes of cell column XW-1
+        // through R[XW/2] at corner-x 0, so it stays inside the box
+        const int col = (k < XW / 2 || XW == 1) ? lane_base : lane_base - gx * LYP;
+        R[k][yb] = lh[col + delta + 2 * k * LYP + yb * 16];
+      }
+#pragma unroll
+    for (int yb = 0; yb < 2; ++yb) {
+      bool yfull = true, yok = true;
+      if (MASKED) {
+        const int ylo = y0 + yb * 16 + oy;  // floor index of row r is ylo + r
+        if (ylo + 15 < 1 || ylo > ny - 2) continue;
+        yfull = ylo >= 1 && ylo + 15 <= ny - 2;
+        yok = (unsigned)(ylo + r - 1) <= (unsigned)(ny - 3);
+      }
+#pragma unroll
+      for (int xc = 0; xc < XW; ++xc) {
+        if (MASKED && (unsigned)(xw0 + xc + ox - 1) > (unsigned)(nx - 3)) continue;
+        uint4 a = R[xc / 2][yb];
+        if (xc & 1) {
+          const uint4 n1 = R[xc / 2 + 1][yb];
+          a.x = gx ? a.x : n1.x;
+          a.y = gx ? a.y : n1.y;
+          a.z = gx ? a.z : n1.z;
+          a.w = gx ? a.w : n1.w;
+        }
+        if (MASKED && !yfull) {
+          a.x = yok ? a.x : 0u;
+          a.y = yok ? a.y : 0u;
+          a.z = yok ? a.z : 0u;
+          a.w = yok ? a.w : 0u;
+        }
+        acc[xc][yb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(fsm_half8, a), (xc & 1) ? b1 : b0,
+                                                             acc[xc][yb], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// MFMA tile shapes and the FP32 tile of the same width they fall back to.
+template <int V> struct MfmaShape;
+template <> struct MfmaShape<0> { static constexpr int XW = 8, M = 4, JS = 1; };
+template <> struct MfmaShape<1> { static constexpr int XW = 2, M = 4, JS = 4; };
+template <> struct MfmaShape<2> { static constexpr int XW = 1, M = 2, JS = 4; };
+
+template <int V>
+__device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, float4 *lds) {
+  constexpr int XW = MfmaShape<V>::XW;
+  constexpr int TX = 4 * XW, TY = 32, NC = TX * TY;
+  const int nx = ev.nx, ny = ev.ny;
+  const int x0 = tile_x * TX, y0 = tile_y * TY;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  {
+    // spline tiles holding the knots this tile can read
+    const int sx0 = max(0, x0 + ev.rbx0) / FSMSP_TX, sx1 = min(nx - 1, x0 + TX - 1 + ev.rbx1) / FSMSP_TX;
+    const int sy0 = max(0, y0 + ev.rby0) / FSMSP_TY, sy1 = min(ny - 1, y0 + TY - 1 + ev.rby1) / FSMSP_TY;
+    const int cy = sy1 - sy0 + 1, cnt = (sx1 - sx0 + 1) * cy;
+    int bad = 0;
+    for (int k = tid; k < cnt; k += blockDim.x) bad |= ev.sp_flags[(sx0 + k / cy) * ev.sp_tiles_y + sy0 + k % cy] != 0u;
+    if (__syncthreads_or(bad)) {
+      stream2d_tile_v2<MfmaShape<V>::M, MfmaShape<V>::JS>(ev, tile_x, tile_y, lds);
+      return;
+    }
+  }
+  const int g = lane >> 4, r = lane & 15, gx = g & 1, gy = g >> 1;
+  uint4 *lh = reinterpret_cast<uint4 *>(lds);
+  fsm_float4v acc[XW][2];
+#pragma unroll
+  for (int xc = 0; xc < XW; ++xc)
+#pragma unroll
+    for (int yb = 0; yb < 2; ++yb) acc[xc][yb] = fsm_float4v{0.f, 0.f, 0.f, 0.f};
+  const int xlast = min(x0 + TX - 1, nx - 1), ylast = min(y0 + TY - 1, ny - 1);
+  for (int gi = 0; gi < ev.n_mgroups; ++gi) {
+    const StencilGroup G = ev.mgroups[gi];
+    // box columns are padded to 16 records (256 B) so that the lanes reading
+    // knot columns c and c+1 do not collide in the LDS banks
+    const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
+    const int no = G.j1 - G.j0;
+    uint4 *bl = lh + LX * LYP;  // [no][64] weights
+    int4 *orec = reinterpret_cast<int4 *>(bl + no * 64);  // [no] (ox, oy, box delta, -)
+    __syncthreads();
+    stage_box<8>(lh, ev.planes_h, x0 + G.bx0, y0 + G.by0, LX, LY, nx, ny, LYP);
+    stage_copy<4>(bl, ev.bmat + (long)G.j0 * 64, no * 64);
+    stage_copy<1>(orec, ev.moffs + G.j0, no);
+    __syncthreads();
+    // every real cell's floor index stays inside [1, n-2] for all offsets of the group
+    // lane's knot (corner) for cell column wave*XW, row r, at box offset 0
+    const int lane_base = (wave * XW + gx) * LYP + r + gy;
+    const bool interior = (x0 + G.bx0 >= 1) && (xlast + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
+                          (ylast + G.by1 - 1 <= ny - 2);
+    if (interior)
+      mfma_offsets<XW, false>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, 0, 0, 0, 0);
+    else
+      mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, x0 + wave * XW, y0, nx, ny);
+  }
+  // moments to LDS: Tt[m][lx*TY + ly]
+  __syncthreads();
+  float *Tt = reinterpret_cast<float *>(lds);
+  {
+    const int n = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int xc = 0; xc < XW; ++xc)
+#pragma unroll
+      for (int yb = 0; yb < 2; ++yb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float v = acc[xc][yb][i];
+          const float w = __shfl_down(v, 8, 16);
+          if (n < 6) Tt[n * NC + (wave * XW + xc) * TY + yb * 16 + q * 4 + i] = v + w;
+        }
+  }
+  __syncthreads();
+  // src/LandauMatch.cpp:104: T = sum * d_phip / TAU; rows 3, 6, 8, 9 carry tanh(0) = 0
+  const long P = ev.t_plane;
+  const float inv_sc = 1.0f / *ev.mscale;  // a power of two: exact
+  constexpr int CPT = (NC + 255) / 256;     // cells per thread
+  float t[CPT][6];
+  int cix[CPT], ciy[CPT];
+  bool live[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int c = tid + 256 * q;
+    const int lx = c / TY, ly = c - lx * TY;
+    cix[q] = x0 + lx;
+    ciy[q] = y0 + ly;
+    live[q] = c < NC && cix[q] < nx && ciy[q] < ny;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) t[q][m] = live[q] ? Tt[m * NC + c] * inv_sc : 0.0f;
+  }
+  // rounding-dependent samples next to the mask boundary: FP32, the
+  // reference's masks (the cells of a thread are evaluated together)
+  for (int a = 0; a < ev.n_amb; ++a) {
+    const Stencil2D S = ev.st[ev.amb[a]];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const float v = live[q] ? sample_correction(ev, S, cix[q], ciy[q]) : 0.0f;
+      t[q][0] += v;
+      t[q][1] = fmaf(v, S.c, t[q][1]);
+      t[q][2] = fmaf(v, S.s, t[q][2]);
+      t[q][3] = fmaf(v, S.cc, t[q][3]);
+      t[q][4] = fmaf(v, S.cs, t[q][4]);
+      t[q][5] = fmaf(v, S.ss, t[q][5]);
+    }
+  }
+  // src/LandauMatch.cpp:104: T = sum * d_phip / TAU; rows 3, 6, 8, 9 carry tanh(0) = 0
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    if (!live[q]) continue;
+    float rr[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) rr[m] = ((t[q][m] * ev.out_scale) * ev.d_phip) / ev.tau;
+    const long is = (long)cix[q] * ny + ciy[q];
+    ev.T[0 * P + is] = rr[0];
+    ev.T[1 * P + is] = rr[1];
+    ev.T[2 * P + is] = rr[2];
+    ev.T[3 * P + is] = 0.0f;
+    ev.T[4 * P + is] = rr[3];
+    ev.T[5 * P + is] = rr[4];
+    ev.T[6 * P + is] = 0.0f;
+    ev.T[7 * P + is] = rr[5];
+    ev.T[8 * P + is] = 0.0f;
+    ev.T[9 * P + is] = 0.0f;
+  }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void k_stream2d_mfma_one(Event2D ev) {
+  extern __shared__ float4 lds4[];
+  const int t = xcd_order(blockIdx.x, gridDim.x);
+  stream2d_mfma_tile<V>(ev, t % ev.tiles_x, t / ev.tiles_x, lds4);
+}
+
 template <int M, int JS>
 __global__ __launch_bounds__(256) void k_stream2d_one(Event2D ev) {
   extern __shared__ float4 lds4[];
-  stream2d_tile_v2<M, JS>(ev, blockIdx.x % ev.tiles_x, blockIdx.x / ev.tiles_x, lds4);
+  const int t = xcd_order(blockIdx.x, gridDim.x);
+  stream2d_tile_v2<M, JS>(ev, t % ev.tiles_x, t / ev.tiles_x, lds4);
 }
 
 // Batched: item = (event, tile x, tile y); each event keeps the tile shape
@@ -207,7 +511,7 @@ __global__ __launch_bounds__(256) void k_stream2d_batch(const Event2D *__restric
                                                         const int4 *__restrict__ items, int n_items) {
   extern __shared__ float4 lds4[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const int4 item = items[it];
+    const int4 item = items[xcd_order(it, n_items)];
     const Event2D ev = evs[item.x];
     switch (ev.variant) {
       case 1: stream2d_tile_v2<4, 4>(ev, item.y, item.z, lds4); break;
@@ -215,6 +519,22 @@ __global__ __launch_bounds__(256) void k_stream2d_batch(const Event2D *__restric
       default: stream2d_tile_v2<4, 1>(ev, item.y, item.z, lds4); break;
     }
     __syncthreads();  // all waves leave the tile before the next item reuses LDS
+  }
+}
+
+// Batched MFMA form (every 2-D event of a batch runs the context's path).
+__global__ __launch_bounds__(256) void k_stream2d_mfma_batch(const Event2D *__restrict__ evs,
+                                                             const int4 *__restrict__ items, int n_items) {
+  extern __shared__ float4 lds4[];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int4 item = items[xcd_order(it, n_items)];
+    const Event2D ev = evs[item.x];
+    switch (ev.variant) {
+      case 1: stream2d_mfma_tile<1>(ev, item.y, item.z, lds4); break;
+      case 2: stream2d_mfma_tile<2>(ev, item.y, item.z, lds4); break;
+      default: stream2d_mfma_tile<0>(ev, item.y, item.z, lds4); break;
+    }
+    __syncthreads();
   }
 }
 
@@ -238,9 +558,26 @@ size_t fsm_stream2d_lds_bytes(int variant, int box_floats4) {
   return box > red ? box : red;
 }
 
+size_t fsm_stream2d_event_lds_bytes(const Event2D &ev) {
+  size_t b = fsm_stream2d_lds_bytes(ev.variant, ev.lds_floats4);
+  if (ev.mfma) {
+    const size_t tile = (size_t)6 * fsm_stream2d_tile_w(ev.variant) * fsm_stream2d_tile_h(ev.variant) * sizeof(float);
+    b = std::max(b, std::max(tile, (size_t)ev.mfma_lds4 * 16));
+  }
+  return b;
+}
+
 hipError_t fsm_launch_stream2d(const Event2D &ev, hipStream_t s) {
   const int tiles = ev.tiles_x * ev.tiles_y;
-  const size_t lds = fsm_stream2d_lds_bytes(ev.variant, ev.lds_floats4);
+  const size_t lds = fsm_stream2d_event_lds_bytes(ev);
+  if (ev.mfma) {
+    switch (ev.variant) {
+      case 1: hipLaunchKernelGGL((k_stream2d_mfma_one<1>), dim3(tiles), dim3(256), lds, s, ev); break;
+      case 2: hipLaunchKernelGGL((k_stream2d_mfma_one<2>), dim3(tiles), dim3(256), lds, s, ev); break;
+      default: hipLaunchKernelGGL((k_stream2d_mfma_one<0>), dim3(tiles), dim3(256), lds, s, ev); break;
+    }
+    return hipGetLastError();
+  }
   switch (ev.variant) {
     case 1: hipLaunchKernelGGL((k_stream2d_one<4, 4>), dim3(tiles), dim3(256), lds, s, ev); break;
     case 2: hipLaunchKernelGGL((k_stream2d_one<2, 4>), dim3(tiles), dim3(256), lds, s, ev); break;
@@ -249,16 +586,22 @@ hipError_t fsm_launch_stream2d(const Event2D &ev, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s) {
+hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int n, size_t lds_bytes, int mfma,
+                                     hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int grid = n < 16384 ? n : 16384;
-  hipLaunchKernelGGL(k_stream2d_batch, dim3(grid), dim3(256), lds_bytes, s, evs, items, n);
+  if (mfma)
+    hipLaunchKernelGGL(k_stream2d_mfma_batch, dim3(grid), dim3(256), lds_bytes, s, evs, items, n);
+  else
+    hipLaunchKernelGGL(k_stream2d_batch, dim3(grid), dim3(256), lds_bytes, s, evs, items, n);
   return hipGetLastError();
 }
 
 hipError_t fsm_stream2d_set_lds_limit(int bytes) {
-  const void *fns[] = {(const void *)k_stream2d_one<4, 1>, (const void *)k_stream2d_one<4, 4>,
-                       (const void *)k_stream2d_one<2, 4>, (const void *)k_stream2d_batch};
+  const void *fns[] = {(const void *)k_stream2d_one<4, 1>,   (const void *)k_stream2d_one<4, 4>,
+                       (const void *)k_stream2d_one<2, 4>,   (const void *)k_stream2d_batch,
+                       (const void *)k_stream2d_mfma_one<0>, (const void *)k_stream2d_mfma_one<1>,
+                       (const void *)k_stream2d_mfma_one<2>, (const void *)k_stream2d_mfma_batch};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;

@@ -46,6 +46,8 @@ FSM_ERR_CANCELLED = -9
 
 PATH_FP32 = 0
 PATH_MIXED = 1
+TILES_AUTO = 0
+TILES_BATCH = 1
 
 _PARAM_FIELDS = [
     ("OUTPUTFORMAT", ctypes.c_int), ("BARYON", ctypes.c_int), ("IC_ENERGY", ctypes.c_int),
@@ -83,6 +85,19 @@ class Params(ctypes.Structure):
 
     def __repr__(self) -> str:  # pragma: no cover
         return "Params(%s)" % ", ".join("%s=%r" % kv for kv in self.as_dict().items())
+
+
+class PlanInfo(ctypes.Structure):
+    """fsm_plan_info: what the planner chose for an event shape."""
+
+    _fields_ = [(n, ctypes.c_int) for n in ("dim3", "variant", "tile_w", "tile_h", "tiles", "n_phi", "n_offsets",
+                                            "n_offset_groups", "n_vector_samples", "lds_bytes", "batch_variant",
+                                            "batch_tiles", "tuned")] + [("tune_us", ctypes.c_float * 3)]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["tune_us"] = [round(float(v), 3) for v in self.tune_us]
+        return d
 
 
 class ServerConfig(ctypes.Structure):
@@ -123,6 +138,10 @@ def _lib():
         lib.fsm_destroy.argtypes = [vp]
         lib.fsm_destroy.restype = None
         lib.fsm_set_path.argtypes = [vp, ctypes.c_int]
+        lib.fsm_plan_describe.argtypes = [vp, P, ctypes.POINTER(PlanInfo)]
+        lib.fsm_set_tile_policy.argtypes = [vp, ctypes.c_int]
+        lib.fsm_set_autotune.argtypes = [vp, ctypes.c_int]
+        lib.fsm_autotune.argtypes = [vp, P, ctypes.POINTER(ctypes.c_int)]
         lib.fsm_run.argtypes = [vp, P, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), fp]
         lib.fsm_run_baryon.argtypes = [vp, P, fp, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), fp]
@@ -294,6 +313,24 @@ class Context:
         n = ctypes.c_int()
         _check(_lib().fsm_kernel_time_totals(self._ctx, ms, ctypes.byref(n)), "fsm_kernel_time_totals")
         return list(ms), n.value
+
+    def set_tile_policy(self, policy: int):
+        """TILES_AUTO (latency shape for one-event calls) or TILES_BATCH."""
+        _check(_lib().fsm_set_tile_policy(self._ctx, policy), "fsm_set_tile_policy")
+
+    def set_autotune(self, on: bool):
+        _check(_lib().fsm_set_autotune(self._ctx, 1 if on else 0), "fsm_set_autotune")
+
+    def autotune(self, p: Params) -> int:
+        v = ctypes.c_int()
+        _check(_lib().fsm_autotune(self._ctx, ctypes.byref(p), ctypes.byref(v)), "fsm_autotune")
+        return v.value
+
+    def plan_info(self, p: Params) -> dict:
+        """fsm_plan_describe for finalized parameters p (on this context's path)."""
+        info = PlanInfo()
+        _check(_lib().fsm_plan_describe(self._ctx, ctypes.byref(p), ctypes.byref(info)), "fsm_plan_describe")
+        return info.as_dict()
 
     def last_kernel_ms(self, which: int) -> float:
         v = ctypes.c_float()
@@ -475,5 +512,6 @@ def exported_symbols_from_header(header_path: str) -> list[str]:
 
 __all__ = [
     "Params", "Context", "FreestreamMilne", "FsmError", "OUTPUT_NAMES", "NUM_OUTPUTS", "params_default",
-    "params_read", "params_finalize", "lib", "version", "LIB_PATH", "PATH_FP32", "PATH_MIXED",
+    "params_read", "params_finalize", "lib", "version", "LIB_PATH", "PATH_FP32", "PATH_MIXED", "TILES_AUTO",
+    "TILES_BATCH",
 ]

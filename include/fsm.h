@@ -121,11 +121,47 @@ void fsm_destroy(fsm_ctx *ctx);
 
 /* Quadrature path selector for the streaming stage. */
 enum {
-  FSM_PATH_FP32 = 0,  /* FP32 VALU interpolation and quadrature */
-  FSM_PATH_MIXED = 1  /* FP32 interpolation, quadrature on FP16 MFMA with
-                         hi+lo split of samples and weights */
+  FSM_PATH_FP32 = 0,  /* FP32 VALU interpolation and quadrature; FP64 Landau */
+  FSM_PATH_MIXED = 1  /* 2-D: interpolation + quadrature as one contraction
+                         on FP16 MFMA (knots and weights split hi+lo, FP32
+                         accumulation), FP32 sample loop where the spline can
+                         undershoot; Landau in FP32 with FP64 for dilute and
+                         high-gamma cells.  No counterpart in the reference. */
 };
 int fsm_set_path(fsm_ctx *ctx, int path);
+
+/* What the planner chose for an event shape (no counterpart in the
+ * reference; for tuning reports).  Plans are cached per parameter set. */
+typedef struct fsm_plan_info {
+  int dim3;             /* 1 for a 3-D (DIM_ETA > 1) plan */
+  int variant;          /* 2-D tile shape index (0 = widest) */
+  int tile_w, tile_h;   /* cells per tile along x and y */
+  int tiles;            /* tiles of one event */
+  int n_phi;            /* azimuth samples */
+  int n_offsets;        /* knot offsets of the MFMA form */
+  int n_offset_groups;  /* LDS stages of the MFMA form */
+  int n_vector_samples; /* samples kept on the FP32 loop in MFMA tiles */
+  int lds_bytes;        /* dynamic LDS per workgroup on the context's path */
+  int batch_variant;    /* tile shape of this event inside batches */
+  int batch_tiles;      /* tiles of one event inside batches */
+  int tuned;            /* 1 if batch_variant was chosen by the autotuner */
+  float tune_us[3];     /* autotuner: stream time per event of each shape */
+} fsm_plan_info;
+int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info);
+
+/* Tile shapes (no counterpart in the reference).  A one-event call uses the
+ * latency shape (enough workgroups for one event to fill the GPU); a batch
+ * uses the plan's batch shape, which the autotuner picks by timing every
+ * shape on a synthetic batch of the event's size the first time the shape
+ * enters a batch or the serving loop (on by default).  FSM_TILES_BATCH makes
+ * one-event calls use the batch shape too, so that a result is bitwise the
+ * same alone and batched. */
+enum { FSM_TILES_AUTO = 0, FSM_TILES_BATCH = 1 };
+int fsm_set_tile_policy(fsm_ctx *ctx, int policy);
+int fsm_set_autotune(fsm_ctx *ctx, int on);
+/* Runs the autotuner for p's shape on the context's path now; returns the
+ * chosen batch shape in *variant (may be NULL). */
+int fsm_autotune(fsm_ctx *ctx, const fsm_params *p, int *variant);
 
 /* ------------------------------------------------------------------ */
 /* Whole step with host buffers (the drop-in call).

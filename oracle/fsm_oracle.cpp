@@ -953,25 +953,48 @@ void oracle_landau(long n, const double *T, float TAU, double *e, double *u) {
 // change of e (kappa_e) and of u^mu / u^tau (kappa_u) over `ntrial` random
 // relative perturbations of size rel of the ten components, divided by rel.
 // T: n x 10 doubles.
-void oracle_landau_condition(long n, const double *T, float TAU, int ntrial, double rel, unsigned seed,
+// normwise = 1: first-order worst case over every component moving by up to
+// rel * T^{tau tau} (times 1/TAU per eta index) -- the size of the rounding
+// error of a quadrature sum of positive samples -- from one-sided differences
+// per component: kappa_e = sum_c |de/dT_c| s_c / e, kappa_u = max_mu
+// sum_c |du^mu/dT_c| s_c / u^tau (in units of rel).  normwise = 0: `ntrial`
+// random perturbations of each component relative to itself.
+void oracle_landau_condition(long n, const double *T, float TAU, int ntrial, double rel, unsigned seed, int normwise,
                              double *kappa_e, double *kappa_u) {
+  static const int eta_idx[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
 #pragma omp parallel for schedule(dynamic, 256)
   for (long i = 0; i < n; ++i) {
     double e0, u0[4];
     landau_cell<double>(T + 10 * i, TAU, &e0, u0);
     double ke = 0.0, ku = 0.0;
-    unsigned s = seed ^ (unsigned)(i * 2654435761u);
-    for (int t = 0; t < ntrial; ++t) {
-      double tp[10];
+    if (normwise) {
+      double se = 0.0, su[4] = {0.0, 0.0, 0.0, 0.0};
       for (int c = 0; c < 10; ++c) {
-        s = s * 1664525u + 1013904223u;
-        double r = ((s >> 8) * (1.0 / 16777216.0)) * 2.0 - 1.0;  // [-1, 1)
-        tp[c] = T[10 * i + c] * (1.0 + rel * r);
+        double tp[10];
+        for (int k = 0; k < 10; ++k) tp[k] = T[10 * i + k];
+        const double h = rel * fabs(T[10 * i]) * pow((double)TAU, -eta_idx[c]);
+        tp[c] += h;
+        double e1, u1[4];
+        landau_cell<double>(tp, TAU, &e1, u1);
+        se += fabs(e1 - e0);
+        for (int m = 0; m < 4; ++m) su[m] += fabs(u1[m] - u0[m]);
       }
-      double e1, u1[4];
-      landau_cell<double>(tp, TAU, &e1, u1);
-      ke = std::max(ke, fabs(e1 - e0) / (fabs(e0) * rel));
-      for (int c = 0; c < 4; ++c) ku = std::max(ku, fabs(u1[c] - u0[c]) / (fabs(u0[0]) * rel));
+      ke = se / (fabs(e0) * rel);
+      for (int m = 0; m < 4; ++m) ku = std::max(ku, su[m] / (fabs(u0[0]) * rel));
+    } else {
+      unsigned s = seed ^ (unsigned)(i * 2654435761u);
+      for (int t = 0; t < ntrial; ++t) {
+        double tp[10];
+        for (int c = 0; c < 10; ++c) {
+          s = s * 1664525u + 1013904223u;
+          double r = ((s >> 8) * (1.0 / 16777216.0)) * 2.0 - 1.0;  // [-1, 1)
+          tp[c] = T[10 * i + c] * (1.0 + rel * r);
+        }
+        double e1, u1[4];
+        landau_cell<double>(tp, TAU, &e1, u1);
+        ke = std::max(ke, fabs(e1 - e0) / (fabs(e0) * rel));
+        for (int c = 0; c < 4; ++c) ku = std::max(ku, fabs(u1[c] - u0[c]) / (fabs(u0[0]) * rel));
+      }
     }
     kappa_e[i] = ke;
     kappa_u[i] = ku;

@@ -149,19 +149,24 @@ def num_threads() -> int:
     return _lib().oracle_num_threads()
 
 
-def landau_condition(T, tau: float, ntrial: int = 4, rel: float = 1e-6, seed: int = 12345):
+def landau_condition(T, tau: float, ntrial: int = 4, rel: float = 1e-6, seed: int = 12345, normwise: bool = True):
     """Per-cell condition numbers (kappa_e, kappa_u) of the Landau matching:
-    max relative change of e (and of u^mu in units of u^tau) per unit relative
-    perturbation of T, over `ntrial` random perturbations.  T: (n, 10)."""
+    relative change of e (and of u^mu in units of u^tau) per unit relative
+    perturbation of T.  T: (n, 10).
+    normwise (default): first-order worst case over every component moving by
+    up to rel * T^{tau tau} (1/tau per eta index), the scale of a quadrature
+    sum's rounding error (one-sided differences per component); otherwise the
+    largest response to `ntrial` random perturbations of each component
+    relative to itself."""
     lib = _lib()
     if not hasattr(lib, "_cond_ready"):
         lib.oracle_landau_condition.argtypes = [ctypes.c_long, ctypes.POINTER(ctypes.c_double), ctypes.c_float,
-                                                ctypes.c_int, ctypes.c_double, ctypes.c_uint,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_uint, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         lib.oracle_landau_condition.restype = None
         lib._cond_ready = True
     T = np.ascontiguousarray(T, dtype=np.float64)
     n = T.shape[0]
     ke, ku = np.empty(n), np.empty(n)
-    lib.oracle_landau_condition(n, _d(T), tau, ntrial, rel, seed, _d(ke), _d(ku))
+    lib.oracle_landau_condition(n, _d(T), tau, ntrial, rel, seed, 1 if normwise else 0, _d(ke), _d(ku))
     return ke, ku

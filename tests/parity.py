@@ -9,13 +9,15 @@ Tolerances (stated by the task for this path):
 
 Conditioning.  Where T^{mu nu} is close to a null tensor (a few streaming
 directions dominate, at grid edges and in dilute cells) the Landau eigenvalue
-is a small difference of large numbers: a relative change of 1e-6 in T --
-the rounding level of the reference's own float sums over N_phi*N_rap
-samples -- can move e by 1e-2 or more.  The reference float path does not
-determine e or u there to 1e-4, so the e/u/pi comparison uses only cells with
-kappa = max |de/e| / |dT/T| <= KAPPA_MAX (kappa from oracle.landau_condition,
-random 1e-6 perturbations in FP64).  The stage-B test compares the GPU Landau
-kernel on identical input T with no such exclusion.
+is a small difference of large numbers: a change of 1e-6 T^{tau tau} in the
+components -- the rounding level of the reference's own float sums over
+N_phi*N_rap positive samples, which is set by T^{tau tau} and not by each
+component's own size -- can move e by 1e-2 or more.  The reference float
+path does not determine e or u there to 1e-4, so the e/u/pi comparison uses
+only cells with kappa = max |de/e| / (|dT| / T^{tau tau}) <= KAPPA_MAX (kappa
+from oracle.landau_condition: first-order worst case over perturbations of
+every component by up to 1e-6 T^{tau tau}, FP64).  The stage-B tests compare the GPU Landau kernels on
+identical input T with no such exclusion.
 """
 import numpy as np
 

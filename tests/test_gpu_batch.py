@@ -20,9 +20,21 @@ def _events():
 
 
 def test_batch_matches_single_events_over_graph_replays(gpu_ctx):
+    """With the batch tile policy a one-event call uses the same tile shape as
+    the batch (the autotuner's choice), so the results are bitwise equal."""
+    evs = _events()
+    gpu_ctx.set_tile_policy(fsm_amd.TILES_BATCH)
+    try:
+        _batch_vs_single(gpu_ctx, evs)
+    finally:
+        gpu_ctx.set_tile_policy(fsm_amd.TILES_AUTO)
+
+
+def _batch_vs_single(gpu_ctx, evs):
     import torch
 
-    evs = _events()
+    for q, _ in evs:
+        gpu_ctx.autotune(q)
     s = torch.cuda.current_stream()
     ins = [torch.from_numpy(e).cuda() for _, e in evs]
     singles = []
@@ -55,6 +67,34 @@ def test_batch_matches_single_events_over_graph_replays(gpu_ctx):
     torch.cuda.synchronize()
     for o, ref in zip(outs, singles):
         assert np.array_equal(o.cpu().numpy(), ref)
+    b.close()
+
+
+def test_batch_within_tolerance_of_latency_shape(gpu_ctx):
+    """Default policy: one-event calls use the latency tile shape, batches the
+    tuned shape; only the summation order differs (e within 1e-5 relative on
+    cells with e > 1e-3 e_max)."""
+    import torch
+
+    evs = _events()
+    s = torch.cuda.current_stream()
+    b = fsm_amd.Batch(gpu_ctx)
+    ins = [torch.from_numpy(e).cuda() for _, e in evs]
+    outs, singles = [], []
+    for (q, _), d in zip(evs, ins):
+        o = torch.empty(17 * q.DIM, dtype=torch.float64, device="cuda")
+        b.add(q, d.data_ptr(), o.data_ptr())
+        outs.append(o)
+        o1 = torch.empty(17 * q.DIM, dtype=torch.float64, device="cuda")
+        gpu_ctx.run_device(q, d.data_ptr(), o1.data_ptr(), s)
+        singles.append(o1)
+    b.run(s)
+    torch.cuda.synchronize()
+    for k, (o, o1) in enumerate(zip(outs, singles)):
+        a, r = o.cpu().numpy().reshape(17, -1), o1.cpu().numpy().reshape(17, -1)
+        m = r[0] > 1e-3 * r[0].max()
+        err = (np.abs(a[0] - r[0])[m] / r[0][m]).max()
+        assert err < 1e-5, (k, err)
     b.close()
 
 
