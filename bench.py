@@ -36,6 +36,26 @@ for p in (os.path.join(ROOT, "freestream-milne_amd"), os.path.join(ROOT, "oracle
 
 FP32_PEAK_TFLOPS = 157.3
 FP16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def load_traffic():
+    """HBM bytes per launch of the stream kernels, measured with rocprofv3
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size) and
+    kept in profiles/traffic_stream2d_c1_batch64.txt as `kernel bytes` lines."""
+    path = os.path.join(ROOT, "profiles", "traffic_stream2d_c1_batch64.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 2 and not line.startswith("#"):
+                try:
+                    out[parts[0]] = int(float(parts[1]))
+                except ValueError:
+                    pass
+    return out
+
+
+TRAFFIC = load_traffic()
 FLOP_PER_SAMPLE_2D = 52  # 16 spline taps * 2 + 10 quadrature rows * 2
 FLOP_PER_SAMPLE_3D = 36  # 8 trilinear taps * 2 + 10 * 2
 
@@ -54,7 +74,43 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only launch what a counter profile needs: a 256 MiB fill and a 256 MiB elementwise "
+                         "copy (the byte-counter calibration), then one eager C1 batch on each path")
     return ap.parse_args()
+
+
+def profile_pass(args):
+    """Short run for rocprofv3 --pmc: plain kernels that move a known number of
+    bytes, then one batch of C1 events on the FP32 and on the mixed path
+    (eager launches, so each stage is one dispatch)."""
+    import torch
+    import fsm_amd
+    import fsm_amd.workloads as W
+
+    n = 64 << 20  # floats: 256 MiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")  # fill kernel: writes 256 MiB
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.mul(a, 2.0, out=b)  # elementwise kernel: reads 256 MiB, writes 256 MiB
+    torch.cuda.synchronize()
+    ctx = fsm_amd.Context(0)
+    p1, e1 = W.config("C1")
+    q1 = fsm_amd.params_finalize(p1, e1)
+    B = args.batch
+    d_in = torch.from_numpy(np.stack([e1] + [W.config("C1", seed=100 + k)[1] for k in range(B - 1)])).cuda()
+    d_out = torch.empty((B, 17, q1.DIM), dtype=torch.float64, device="cuda")
+    for path in (fsm_amd.PATH_FP32, fsm_amd.PATH_MIXED):
+        ctx.set_path(path)
+        bt = fsm_amd.Batch(ctx)
+        bt.set_graph(False)
+        for k in range(B):
+            bt.add(q1, d_in[k].data_ptr(), d_out[k].data_ptr())
+        bt.run(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        bt.close()
+    ctx.close()
+    print(json.dumps({"profile_pass": True, "copy_bytes": 4 * n, "batch": B}), flush=True)
 
 
 def timed(fn, steps, sync):
@@ -120,6 +176,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(events shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.profile_pass:
+        profile_pass(args)
+        return
     import torch
     import fsm_amd
     import fsm_amd.workloads as W
@@ -184,7 +243,9 @@ def main():
     info = ctx.plan_info(q1)  # context is on the mixed path here
     samples = D * q1.DIM_PHIP * q1.DIM_RAP * B
     k_ms = mixed["stream_launch_ms"]
-    mfma_flops = B * info["tiles"] * (info["tile_w"] * info["tile_h"] // 16) * info["n_offsets"] * 16384
+    # MFMA form: per (cell, knot offset) one 16x16x32 product row, i.e.
+    # 16*32*2 = 1024 FLOP per cell and offset (12 of the 16 columns are used)
+    mfma_flops = B * D * info["n_offsets"] * 1024
     achieved = mfma_flops / (k_ms * 1e-3) / 1e12
     roofline = {
         "kernel": "k_stream2d_mfma_batch",
@@ -193,18 +254,18 @@ def main():
         "peak": FP16_MFMA_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4),
-        "traffic": None,
+        "traffic": TRAFFIC.get("k_stream2d_mfma_batch"),
         "avg_kernel_ms": round(k_ms, 5),
         "flop_per_launch": mfma_flops,
         "fp32_equivalent_tflops": round(FLOP_PER_SAMPLE_2D * samples / (k_ms * 1e-3) / 1e12, 3),
-        "note": "v_mfma_f32_16x16x32_f16 issue: %d events x %d tiles x %d cell blocks x %d knot offsets x "
-                "16384 FLOP per launch, against MI355X's dense FP16 peak; fp32_equivalent counts the "
-                "reference's 52 FLOP per (cell, phi) sample over the same time" %
-                (B, info["tiles"], info["tile_w"] * info["tile_h"] // 16, info["n_offsets"]),
+        "note": "v_mfma_f32_16x16x32_f16 work of the knot-offset form: %d events x %d cells x %d knot offsets "
+                "x 1024 FLOP per launch, against MI355X's dense FP16 peak; fp32_equivalent counts the "
+                "reference's 52 FLOP per (cell, phi) sample over the same time; traffic = HBM bytes per "
+                "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)" % (B, D, info["n_offsets"]),
     }
     kf = fp32["stream_launch_ms"]
     roofline_fp32 = {
-        "kernel": "k_stream2d_batch", "bound": "valu",
+        "kernel": "k_stream2d_batch", "bound": "valu", "traffic": TRAFFIC.get("k_stream2d_batch"),
         "achieved": round(FLOP_PER_SAMPLE_2D * samples / (kf * 1e-3) / 1e12, 3),
         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(FLOP_PER_SAMPLE_2D * samples / (kf * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),

@@ -143,7 +143,22 @@ struct Plan {
   DevBuf<float> cs;
   DevBuf<float> src;
   DevBuf<float> Tem;  // eta-major T
+  // charge pass (BARYON = 1): normalisation of convertInitialChargeDensity;
+  // 3-D: rapidity slots with the charge rapidity factors, source planes, J
+  float charge_norm = 0.0f;
+  Event3D ev3b{};
+  DevBuf<RapSlot> slots_b;
+  DevBuf<float> src_b;
+  DevBuf<float> Jem;  // eta-major J (rows 0-3 of a 10-plane T)
 };
+
+// src/FreeStream.cpp:365-366 with the reference's float/double mixing
+// (SIGMA_B * SIGMA_B is a float product).
+float charge_norm_factor(float SB) {
+  const float sb2 = SB * SB;
+  const float n = (float)(sqrt(2.0 * M_PI) * (double)SB * exp((double)sb2 / 2.0));
+  return (float)(1.0 / (2.0 * M_PI * (double)n));
+}
 
 bool same_shape(const fsm_params &a, const fsm_params &b) { return memcmp(&a, &b, sizeof(a)) == 0; }
 
@@ -173,7 +188,7 @@ struct fsm_ctx {
   std::mutex plan_mu;  // the plan cache is shared with the server's worker thread
   std::mutex tune_mu;  // one autotuning run at a time
   // host-API workspace
-  DevBuf<float> d_in, d_T;
+  DevBuf<float> d_in, d_nb, d_T, d_J;
   DevBuf<double> d_out;
 };
 
@@ -196,6 +211,10 @@ int check_supported(const fsm_params &p) {
   if (!(p.DX > 0) || !(p.DY > 0) || !(p.TAU > 0)) return FSM_ERR_UNSUPPORTED;
   if (p.DIM != p.DIM_X * p.DIM_Y * p.DIM_ETA) return FSM_ERR_ARG;
   if (p.DIM_ETA > 1 && (p.DIM_ETA < 3 || !(p.DETA > 0) || !(p.TAU0 > 0))) return FSM_ERR_UNSUPPORTED;
+  // 2-D charge pass: the reference builds the 2-D spline from rapidity copy 0
+  // only (src/FreeStream.cpp:121-153), which for DIM_RAP > 1 differs from the
+  // per-copy charge densities; only DIM_RAP = 1 is supported there
+  if (p.BARYON && p.DIM_ETA == 1 && p.DIM_RAP != 1) return FSM_ERR_UNSUPPORTED;
   return FSM_OK;
 }
 
@@ -415,7 +434,7 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   const int stx = (nx + FSMSP_TX - 1) / FSMSP_TX, sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
   FSM_TRY(pl.planes_h.ensure((size_t)nx * ny));
   FSM_TRY(pl.sp_flags.ensure((size_t)stx * sty));
-  FSM_TRY(pl.mscale.ensure(1));
+  FSM_TRY(pl.mscale.ensure(FSM_KS_PARTS + 1));
   Event2D &ev = pl.ev2;
   memset(&ev, 0, sizeof(ev));
   ev.nx = nx;
@@ -460,6 +479,11 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.planes_h = nullptr;
   ev.sp_flags = pl.sp_flags.p;
   ev.mscale = pl.mscale.p;
+  pl.charge_norm = charge_norm_factor(p.SIGMA_B);
+  ev.charge = 0;
+  ev.in_norm = pl.charge_norm;
+  ev.out_j = 0;
+  ev.jscale = p.DRAP;
   ev.moffs = pl.moffs.p;
   ev.bmat = pl.bmat.p;
   ev.mgroups = pl.mgroups.p;
@@ -561,6 +585,39 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     float nf = (float)n;
     ev.norm_factor = (float)(1.0 / (2.0 * M_PI * (double)nf));
   }
+  ev.charge = 0;
+  if (p.BARYON) {
+    // charge pass: same geometry; the density carries the charge rapidity
+    // factor on the uniform DRAP grid (src/FreeStream.cpp:368-385) and
+    // calculateBaryonCurrent weights every slot by DRAP (src/LandauMatch.cpp:140)
+    const float SB = p.SIGMA_B, DRAP = p.DRAP, sb2 = SB * SB;
+    const float rapmin = (float)((-1.0) * ((double)(float)(nrap - 1) / 2.0) * (double)DRAP);
+    auto crf = [&](int ie, int k) {
+      const float eta = (float)ie * DETA + etamin;
+      const float rap = (float)k * DRAP + rapmin;
+      const float d = eta - rap;
+      return (float)((double)coshf(d) * exp((-1.0) * (double)d * (double)d / (double)sb2));
+    };
+    std::vector<RapSlot> sb(slots);
+    for (int ie = 0; ie < neta; ++ie)
+      for (int k = 0; k < nrap; ++k) {
+        RapSlot &S = sb[(size_t)ie * nrap + k];
+        if (!S.valid) continue;
+        S.rf0 = crf(S.eta0, k);
+        S.rf1 = crf(S.eta0 + 1, k);
+        S.jac = DRAP;
+      }
+    FSM_TRY(pl.slots_b.upload(sb));
+    FSM_TRY(pl.src_b.ensure((size_t)p.DIM));
+    FSM_TRY(pl.Jem.ensure((size_t)10 * p.DIM));
+    pl.charge_norm = charge_norm_factor(SB);
+    pl.ev3b = ev;
+    pl.ev3b.charge = 1;
+    pl.ev3b.norm_factor = pl.charge_norm;
+    pl.ev3b.src = pl.src_b.p;
+    pl.ev3b.slots = pl.slots_b.p;
+    pl.ev3b.T = pl.Jem.p;
+  }
   return FSM_OK;
 }
 
@@ -651,6 +708,38 @@ int stage_a(fsm_ctx *ctx, Plan &pl, const fsm_params &p, const float *d_e, float
   return FSM_OK;
 }
 
+// Charge pass of the baryon extension: n_B through convertInitialChargeDensity
+// and freeStream into J^mu (calculateBaryonCurrent), in the plan's buffers
+// after the energy pass on the same stream.  2-D: J goes to J_ref (4 planes,
+// reference layout); 3-D: to the plan's eta-major buffer.
+int stream_charge(fsm_ctx *ctx, Plan &pl, const fsm_params &p, const float *d_nb, float *J_ref, hipStream_t s,
+                  const float **J_used) {
+  if (!pl.is3d) {
+    Event2D ev = pl.ev2;
+    ev.e_gev = d_nb;
+    ev.T = J_ref;
+    ev.charge = 1;
+    ev.out_j = 1;
+    ev.out_scale = 1.0f;  // DIM_RAP = 1
+    if (ctx->tile_policy == FSM_TILES_BATCH) set_variant(ev, pl.bvariant[ctx->path]);
+    if (ctx->path == FSM_PATH_MIXED) {
+      ev.mfma = 1;
+      ev.planes_h = pl.planes_h.p;
+    }
+    FSM_TRY(fsm_launch_spline2d(ev, s));
+    FSM_TRY(fsm_launch_stream2d(ev, s));
+    *J_used = J_ref;
+  } else {
+    Event3D ev = pl.ev3b;
+    ev.e_gev = d_nb;
+    FSM_TRY(fsm_launch_src3d(ev, s));
+    FSM_TRY(fsm_launch_stream3d(ev, s));
+    *J_used = pl.Jem.p;
+  }
+  (void)p;
+  return FSM_OK;
+}
+
 EventLandau landau_event(const fsm_params &p, const float *T, int eta_major, int mixed) {
   EventLandau L;
   memset(&L, 0, sizeof(L));
@@ -682,7 +771,7 @@ void finish_timing(fsm_ctx *ctx) {
 int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p);  // batch section
 
 int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double *d_out, float *T_scratch,
-                    hipStream_t s) {
+                    hipStream_t s, const float *d_nb = nullptr, float *J_scratch = nullptr) {
   Plan *pl = nullptr;
   int st = get_plan(ctx, *p, &pl);
   if (st) return st;
@@ -690,8 +779,17 @@ int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double 
   int em = 0;
   st = stage_a(ctx, *pl, *p, d_e, T_scratch, s, &T, &em);
   if (st) return st;
+  const float *J = nullptr;
+  if (d_nb) {
+    st = stream_charge(ctx, *pl, *p, d_nb, J_scratch, s, &J);
+    if (st) return st;
+  }
   EventLandau L = landau_event(*p, T, em, ctx->path == FSM_PATH_MIXED);
   L.out = d_out;
+  if (J) {
+    L.J = J;
+    L.out_b = d_out + (size_t)FSM_NUM_OUTPUTS * p->DIM;
+  }
   FSM_TRY(fsm_launch_landau(L, s));
   tmark(ctx, 3, s);
   return FSM_OK;
@@ -922,15 +1020,55 @@ int fsm_run(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, size_t n, double *c
   return FSM_OK;
 }
 
+int fsm_run_baryon_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, const float *d_nB,
+                          double *d_out, void *stream) {
+  if (!ctx || !p || !d_e_GeV || !d_out) return FSM_ERR_ARG;
+  if (!p->BARYON || !d_nB) return fsm_run_device(ctx, p, d_e_GeV, d_out, stream);
+  int st = check_supported(*p);
+  if (st) return st;
+  FSM_TRY(hipSetDevice(ctx->device));
+  float *Tscr = nullptr, *Jscr = nullptr;
+  if (p->DIM_ETA == 1) {
+    FSM_TRY(ctx->d_T.ensure((size_t)10 * p->DIM));
+    FSM_TRY(ctx->d_J.ensure((size_t)4 * p->DIM));
+    Tscr = ctx->d_T.p;
+    Jscr = ctx->d_J.p;
+  }
+  return run_device_impl(ctx, p, d_e_GeV, d_out, Tscr, (hipStream_t)stream, d_nB, Jscr);
+}
+
 int fsm_run_baryon(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, const float *nB_in, size_t n,
                    double *const out[FSM_NUM_OUTPUTS], double *nB_out, double *const V_out[4],
                    float *tau_landau_match) {
-  (void)nB_in;
-  (void)nB_out;
-  (void)V_out;
-  if (!p) return FSM_ERR_ARG;
-  if (p->BARYON) return FSM_ERR_UNSUPPORTED;
-  return fsm_run(ctx, p, e_GeV, n, out, tau_landau_match);
+  if (!ctx || !p || !e_GeV || !out) return FSM_ERR_ARG;
+  if (!p->BARYON) return fsm_run(ctx, p, e_GeV, n, out, tau_landau_match);
+  if (!nB_in) return FSM_ERR_ARG;
+  for (int k = 0; k < FSM_NUM_OUTPUTS; ++k)
+    if (!out[k]) return FSM_ERR_ARG;
+  int st = fsm_params_finalize(p, e_GeV, n);
+  if (st) return st;
+  if (tau_landau_match) *tau_landau_match = p->TAU;
+  FSM_TRY(hipSetDevice(ctx->device));
+  const size_t D = (size_t)p->DIM;
+  FSM_TRY(ctx->d_in.ensure(D));
+  FSM_TRY(ctx->d_nb.ensure(D));
+  FSM_TRY(ctx->d_out.ensure(D * FSM_NUM_OUTPUTS_BARYON));
+  hipStream_t s = nullptr;
+  FSM_TRY(hipMemcpyAsync(ctx->d_in.p, e_GeV, D * sizeof(float), hipMemcpyHostToDevice, s));
+  FSM_TRY(hipMemcpyAsync(ctx->d_nb.p, nB_in, D * sizeof(float), hipMemcpyHostToDevice, s));
+  st = fsm_run_baryon_device(ctx, p, ctx->d_in.p, ctx->d_nb.p, ctx->d_out.p, s);
+  if (st) return st;
+  for (int k = 0; k < FSM_NUM_OUTPUTS; ++k)
+    FSM_TRY(hipMemcpyAsync(out[k], ctx->d_out.p + k * D, D * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (nB_out)
+    FSM_TRY(hipMemcpyAsync(nB_out, ctx->d_out.p + FSM_NUM_OUTPUTS * D, D * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (V_out)
+    for (int c = 0; c < 4; ++c)
+      if (V_out[c])
+        FSM_TRY(hipMemcpyAsync(V_out[c], ctx->d_out.p + (FSM_NUM_OUTPUTS + 1 + c) * D, D * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  FSM_TRY(hipStreamSynchronize(s));
+  return FSM_OK;
 }
 
 }  // extern "C"
@@ -944,6 +1082,7 @@ struct BatchEvent {
   std::shared_ptr<Plan> plan;
   fsm_params p;
   const float *d_e;
+  const float *d_nb = nullptr;  // baryon events: initial n_B (charge pass)
   double *d_out;
   char *ws = nullptr;  // caller-provided workspace (event_workspace_bytes), else the batch arena
 };
@@ -956,11 +1095,18 @@ size_t spline_flag_count(const fsm_params &p) {
   return (size_t)((p.DIM_X + FSMSP_TX - 1) / FSMSP_TX) * ((p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY);
 }
 
+size_t event_workspace_bytes(const fsm_params &p);
+
+// Baryon events stream twice (energy and charge), each pass with its own workspace.
+size_t event_workspace_bytes_all(const fsm_params &p, bool baryon) {
+  return event_workspace_bytes(p) * (baryon ? 2 : 1);
+}
+
 size_t event_workspace_bytes(const fsm_params &p) {
   if (p.DIM_ETA == 1)
     return align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4)) + align256((size_t)10 * p.DIM * sizeof(float)) +
            align256((size_t)p.DIM_X * p.DIM_Y * sizeof(uint4)) + align256(spline_flag_count(p) * sizeof(unsigned)) +
-           align256(sizeof(float));
+           align256((FSM_KS_PARTS + 1) * sizeof(float));
   return align256((size_t)p.DIM * sizeof(float)) + align256((size_t)10 * p.DIM * sizeof(float));
 }
 
@@ -982,7 +1128,7 @@ Event2D event2d_in_workspace(const Plan &pl, const fsm_params &p, const float *d
   d.sp_flags = (unsigned *)cur;
   cur += align256(spline_flag_count(p) * sizeof(unsigned));
   d.mscale = (float *)cur;
-  cur += align256(sizeof(float));
+  cur += align256((FSM_KS_PARTS + 1) * sizeof(float));
   d.mfma = path == FSM_PATH_MIXED ? 1 : 0;
   d.planes_h = d.mfma ? ph : nullptr;
   return d;
@@ -1026,7 +1172,8 @@ int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p) {
     const int stx = (nx + FSMSP_TX - 1) / FSMSP_TX, sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
     for (int i = 0; i < n; ++i) {
       Event2D d = event2d_in_workspace(pl, p, d_in.p, path, v, cur);
-      if (d.mfma) ks.push_back(make_int4(i, 0, 0, 0));
+      if (d.mfma)
+        for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
       for (int x = 0; x < stx; ++x)
         for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
       for (int y = 0; y < d.tiles_y; ++y)
@@ -1146,53 +1293,70 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   std::vector<int4> ks, sp, s2, src, s3, ld;
   size_t bytes = 0;
   for (auto &ev : b->evs)
-    if (!ev.ws) bytes += event_workspace_bytes(ev.p);
+    if (!ev.ws) bytes += event_workspace_bytes_all(ev.p, ev.d_nb != nullptr);
   if (bytes) FSM_TRY(b->arena.ensure(bytes));
   char *arena_cur = b->arena.p;
   b->lds2 = b->lds3 = 0;
+  const int path = b->ctx->path;
+  // stage-A work items of one streaming pass (energy, or the charge pass of a
+  // baryon event) in the workspace at `cur`; returns its T (or J) and layout
+  auto add_pass = [&](const BatchEvent &ev, const float *d_in, bool charge, char *&cur, int *eta_major) {
+    const fsm_params &p = ev.p;
+    if (!ev.plan->is3d) {
+      Event2D d = event2d_in_workspace(*ev.plan, p, d_in, path, ev.plan->bvariant[path], cur);
+      if (charge) {
+        d.charge = 1;
+        d.out_j = 1;
+        d.out_scale = 1.0f;  // DIM_RAP = 1
+      }
+      const int i = (int)e2.size();
+      if (d.mfma)
+        for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
+      e2.push_back(d);
+      const int stx = (p.DIM_X + FSMSP_TX - 1) / FSMSP_TX, sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
+      for (int x = 0; x < stx; ++x)
+        for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
+      for (int y = 0; y < d.tiles_y; ++y)
+        for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
+      b->lds2 = std::max(b->lds2, fsm_stream2d_event_lds_bytes(d));
+      *eta_major = 0;
+      return (const float *)d.T;
+    }
+    Event3D d = charge ? ev.plan->ev3b : ev.plan->ev3;
+    d.e_gev = d_in;
+    d.src = (float *)cur;
+    cur += align256((size_t)p.DIM * sizeof(float));
+    d.T = (float *)cur;
+    cur += align256((size_t)10 * p.DIM * sizeof(float));
+    d.t_plane = p.DIM;
+    const int i = (int)e3.size();
+    e3.push_back(d);
+    const int nxy = p.DIM_X * p.DIM_Y;
+    for (int x = 0; x < (nxy + 63) / 64; ++x)
+      for (int y = 0; y < (p.DIM_ETA + 63) / 64; ++y) src.push_back(make_int4(i, x, y, 0));
+    for (int ie = 0; ie < p.DIM_ETA; ++ie)
+      for (int t = 0; t < d.tiles; ++t) s3.push_back(make_int4(i, t, ie, 0));
+    b->lds3 = std::max(b->lds3, (size_t)d.lds_floats * sizeof(float));
+    *eta_major = 1;
+    return (const float *)d.T;
+  };
   for (auto &ev : b->evs) {
     const fsm_params &p = ev.p;
     const int li = (int)el.size();
     char *cur = ev.ws;
     if (!cur) {
       cur = arena_cur;
-      arena_cur += event_workspace_bytes(p);
+      arena_cur += event_workspace_bytes_all(p, ev.d_nb != nullptr);
     }
-    if (!ev.plan->is3d) {
-      Event2D d = event2d_in_workspace(*ev.plan, p, ev.d_e, b->ctx->path, ev.plan->bvariant[b->ctx->path], cur);
-      const int i = (int)e2.size();
-      if (d.mfma) ks.push_back(make_int4(i, 0, 0, 0));
-      e2.push_back(d);
-      const int stx = (p.DIM_X + FSMSP_TX - 1) / FSMSP_TX, sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
-      for (int x = 0; x < stx; ++x)
-        for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
-      // each event keeps its plan's tile shape (identical bits alone and batched)
-      for (int y = 0; y < d.tiles_y; ++y)
-        for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
-      b->lds2 = std::max(b->lds2, fsm_stream2d_event_lds_bytes(d));
-      EventLandau L = landau_event(p, d.T, 0, b->ctx->path == FSM_PATH_MIXED);
-      L.out = ev.d_out;
-      el.push_back(L);
-    } else {
-      Event3D d = ev.plan->ev3;
-      d.e_gev = ev.d_e;
-      d.src = (float *)cur;
-      cur += align256((size_t)p.DIM * sizeof(float));
-      d.T = (float *)cur;
-      cur += align256((size_t)10 * p.DIM * sizeof(float));
-      d.t_plane = p.DIM;
-      const int i = (int)e3.size();
-      e3.push_back(d);
-      const int nxy = p.DIM_X * p.DIM_Y;
-      for (int x = 0; x < (nxy + 63) / 64; ++x)
-        for (int y = 0; y < (p.DIM_ETA + 63) / 64; ++y) src.push_back(make_int4(i, x, y, 0));
-      for (int ie = 0; ie < p.DIM_ETA; ++ie)
-        for (int t = 0; t < d.tiles; ++t) s3.push_back(make_int4(i, t, ie, 0));
-      b->lds3 = std::max(b->lds3, (size_t)d.lds_floats * sizeof(float));
-      EventLandau L = landau_event(p, d.T, 1, b->ctx->path == FSM_PATH_MIXED);
-      L.out = ev.d_out;
-      el.push_back(L);
+    int em = 0, jm = 0;
+    const float *T = add_pass(ev, ev.d_e, false, cur, &em);
+    EventLandau L = landau_event(p, T, em, path == FSM_PATH_MIXED);
+    L.out = ev.d_out;
+    if (ev.d_nb) {
+      L.J = add_pass(ev, ev.d_nb, true, cur, &jm);
+      L.out_b = ev.d_out + (size_t)FSM_NUM_OUTPUTS * p.DIM;
     }
+    el.push_back(L);
     for (int c = 0; c < p.DIM; c += kLandauChunk) ld.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
   }
   std::vector<int4> items;
@@ -1287,6 +1451,17 @@ int fsm_batch_add(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, doubl
   return (int)b->evs.size() - 1;
 }
 
+int fsm_batch_add_baryon(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, const float *d_nB,
+                         double *d_out) {
+  if (!b || !p || !d_nB) return FSM_ERR_ARG;
+  if (!p->BARYON) return FSM_ERR_ARG;
+  int st = check_supported(*p);
+  if (st) return st;
+  int i = fsm_batch_add(b, p, d_e_GeV, d_out);
+  if (i >= 0) b->evs[i].d_nb = d_nB;
+  return i;
+}
+
 int fsm_batch_clear(fsm_batch *b) {
   if (!b) return FSM_ERR_ARG;
   b->evs.clear();
@@ -1335,10 +1510,10 @@ int fsm_batch_run(fsm_batch *b, void *stream) {
 // ---- internal entry points for the server (fsm_host.h) ----------------------
 namespace fsm_host {
 
-size_t workspace_bytes(const fsm_params &p) { return event_workspace_bytes(p); }
+size_t workspace_bytes(const fsm_params &p, bool baryon) { return event_workspace_bytes_all(p, baryon); }
 
-int batch_add_ws(fsm_batch *b, const fsm_params *p, const float *d_e, double *d_out, void *ws) {
-  int i = fsm_batch_add(b, p, d_e, d_out);
+int batch_add_ws(fsm_batch *b, const fsm_params *p, const float *d_e, const float *d_nb, double *d_out, void *ws) {
+  int i = d_nb ? fsm_batch_add_baryon(b, p, d_e, d_nb, d_out) : fsm_batch_add(b, p, d_e, d_out);
   if (i >= 0) b->evs[i].ws = (char *)ws;
   return i;
 }

@@ -11,10 +11,11 @@
 namespace fsm_host {
 
 // Device workspace one event needs inside a batch (spline or source planes
-// plus the stress tensor).
-size_t workspace_bytes(const fsm_params &p);
-// fsm_batch_add with caller-owned workspace of workspace_bytes(*p).
-int batch_add_ws(fsm_batch *b, const fsm_params *p, const float *d_e, double *d_out, void *ws);
+// plus the stress tensor; twice that for a baryon event's two passes).
+size_t workspace_bytes(const fsm_params &p, bool baryon);
+// fsm_batch_add (d_nb == NULL) or fsm_batch_add_baryon with caller-owned
+// workspace of workspace_bytes(*p, d_nb != NULL).
+int batch_add_ws(fsm_batch *b, const fsm_params *p, const float *d_e, const float *d_nb, double *d_out, void *ws);
 // Descriptor uploads through pinned staging, ordered on the launch stream.
 void batch_set_async_upload(fsm_batch *b, bool on);
 // Builds (if changed) and launches the batch on `s` without a graph.

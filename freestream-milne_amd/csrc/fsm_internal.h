@@ -18,6 +18,27 @@
 
 #define FSM_HBARC_F 0.197326938f
 
+// The reference's keep/drop decision for a shifted sample
+// (src/FreeStream.cpp:163,187,198,204,213) with its own float operations and
+// rounding: x = (float)i*D + xmin; x_new = x - shift; f = floor((x_new -
+// xmin)/D); kept iff f in [1, n-2].  Contraction is off here: HIP's
+// __fadd_rn/__fmul_rn are plain operators that the compiler may fuse into an
+// FMA, which changes the decision when a shift is a whole number of cells.
+__device__ __forceinline__ bool fsm_ref_mask_ok(int i, float shift, float mn, float h, int n) {
+#pragma clang fp contract(off)
+  const float x = (float)i * h + mn;
+  const float xn = x - shift;
+  const float f = floorf((xn - mn) / h);
+  return (f >= 1.0f) && (f < (float)(n - 1));
+}
+
+// The same decision for a sample whose constant-stencil floor index is g:
+// only g within one cell of the mask boundary can round either way.
+__device__ __forceinline__ bool fsm_mask_ok(int i, int g, float shift, float mn, float h, int n) {
+  if ((unsigned)g <= 1u || (unsigned)(g - (n - 2)) <= 1u) return fsm_ref_mask_ok(i, shift, mn, h, n);
+  return (unsigned)(g - 1) <= (unsigned)(n - 3);
+}
+
 // XCD-aware work order.  An MI355X dispatches workgroups to its 8 XCDs round
 // robin (workgroup b on XCD b % 8), and each XCD has its own L2.  Sending a
 // contiguous range of work items to each XCD keeps neighbouring tiles, which
@@ -102,11 +123,16 @@ struct Event2D {
   int sp_tiles_y;       // spline tiles along y (flag index = sx * sp_tiles_y + sy)
   uint4 *planes_h;      // per knot 8 halves: scale*{z, zy, zx, zxy} hi, then lo
   unsigned *sp_flags;   // per spline tile: nonzero if a knot is unsafe for the MFMA form
-  float *mscale;        // per event: power-of-two knot scale of planes_h (k_knot_scale)
+  float *mscale;        // per event: FSM_KS_PARTS partial max |z|, then the power-of-two knot scale
   const int4 *moffs;    // [n_moff] (ox, oy, LDS delta of the offset in its group's knot box, -)
   const uint4 *bmat;    // [n_moff][64] MFMA B operand per lane (8 halves)
   const StencilGroup *mgroups;
   const int *amb;       // [n_amb] indices into st
+  // ---- charge pass of the baryon extension ----
+  int charge;           // 1: the input is n_B (fm^-3) and z = n_B * in_norm (convertInitialChargeDensity)
+  float in_norm;        // its normalisation (src/FreeStream.cpp:365-366)
+  int out_j;            // 1: write J^tau, J^x, J^y, J^eta (4 planes of T) instead of T^{mu nu}
+  float jscale;         // DRAP of calculateBaryonCurrent (src/LandauMatch.cpp:140)
 };
 
 // The MFMA form of the 2-D quadrature.  For all samples j sharing a knot
@@ -124,6 +150,7 @@ struct Event2D {
 // (Bernstein control points, where the reference falls back to bilinear),
 // and MFMA tiles that read a flagged knot run the FP32 sample loop instead.
 #define FSM_MFMA_ZTARGET 4096.0f
+#define FSM_KS_PARTS 64
 #define FSM_MFMA_MAXABS 30000.0f
 #define FSM_MFMA_POS_EPS 1.0e-5f
 
@@ -165,8 +192,8 @@ struct Event3D {
   const float *cs;      // [2*nphi]: cos(phi_j), sin(phi_j)
   float *T;             // 10 planes, eta-major layout
   long t_plane;
-  float norm_factor;
-  float pad;
+  float norm_factor;    // energy: convertInitialDensity norm; charge pass: convertInitialChargeDensity norm
+  int charge;           // 1: the input is n_B: src = n_B * norm_factor (no hbarc, no floor)
 };
 
 // --------------------------------------------------------------- Landau ----
@@ -184,6 +211,8 @@ struct EventLandau {
   double *out;          // 17 planes of dim doubles (may be null)
   long out_plane;
   float *fe, *fu, *fp, *fPi, *fpi;  // optional float outputs
+  const float *J;       // baryon current J^tau, J^x, J^y, J^eta (T's layout and plane stride), or null
+  double *out_b;        // n_B, V^tau, V^x, V^y, V^eta: 5 planes of out_plane doubles (with J)
 };
 
 #endif  // FSM_INTERNAL_H_

@@ -37,22 +37,13 @@ __device__ __forceinline__ float ingest_e0(float v) {
   return v * 1.0f / FSM_HBARC_F + 1.0e-7f;
 }
 
-// Mask decision of the reference for a sample whose shifted index lies next
-// to the mask boundary (src/FreeStream.cpp:163,187,198,204,213), evaluated
-// with the reference's own float operations and rounding:
-//   x = (float)i*D + xmin;  x_new = x - shift;  f = floor((x_new - xmin)/D)
-// and kept iff f in [1, n-2].  Away from the boundary the per-phi stencil
-// offset decides, since a one-ulp difference cannot move the floor across it.
-__device__ __forceinline__ bool ref_mask_ok(int i, float shift, float mn, float h, int n) {
-  const float x = __fadd_rn(__fmul_rn((float)i, h), mn);
-  const float xn = __fsub_rn(x, shift);
-  const float f = floorf(__fdiv_rn(__fsub_rn(xn, mn), h));
-  return (f >= 1.0f) && (f < (float)(n - 1));
-}
-
-__device__ __forceinline__ bool mask_ok(int i, int g, float shift, float mn, float h, int n) {
-  if ((unsigned)g <= 1u || (unsigned)(g - (n - 2)) <= 1u) return ref_mask_ok(i, shift, mn, h, n);
-  return (unsigned)(g - 1) <= (unsigned)(n - 3);
+// Knot value of the 2-D spline for one input value: the energy path's
+// e0 / (2 pi) (src/FreeStream.cpp:318) or the charge path's n_B * norm
+// (src/FreeStream.cpp:384 with a rapidity factor of 1 at eta = rap = 0; the
+// sum over rapidity copies is folded into the output scale).
+__device__ __forceinline__ float knot_z(const Event2D &ev, float v) {
+  if (ev.charge) return v * ev.in_norm;
+  return (float)((double)ingest_e0(v) / (2.0 * M_PI));
 }
 
 // ======================================================================
@@ -70,6 +61,7 @@ struct SplineLds {
   float wx[FSMSP_TX * FSM_SPLINE_W];
   float wy[FSMSP_TY * FSM_SPLINE_W];
   float red[4];
+  float sc;  // knot scale of the MFMA planes
 };
 
 // Halves of the MFMA knot format: h = fp16(x), l = fp16(x - h).
@@ -92,6 +84,8 @@ __device__ __forceinline__ float knot_cp_min(float z, float zy, float zx, float 
   return m;
 }
 
+__device__ float knot_scale_from_parts(const Event2D &ev);  // below, with k_knot_scale
+
 __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
   const int nx = ev.nx, ny = ev.ny;
   const int x0 = tx * FSMSP_TX, y0 = ty * FSMSP_TY;
@@ -102,10 +96,7 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
     int lx = k / SP_RY, ly = k - lx * SP_RY;
     int gx = x0 - B + lx, gy = y0 - B + ly;
     float z = 0.0f;
-    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
-      float e0 = ingest_e0(ev.e_gev[(long)gx * ny + gy]);
-      z = (float)((double)e0 / (2.0 * M_PI));  // src/FreeStream.cpp:318
-    }
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) z = knot_z(ev, ev.e_gev[(long)gx * ny + gy]);
     L.zr[k] = z;
     zabs = fmaxf(zabs, fabsf(z));
   }
@@ -114,6 +105,13 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
     // noise in the knot slopes, below which a negative control point is noise
     for (int o = 32; o > 0; o >>= 1) zabs = fmaxf(zabs, __shfl_xor(zabs, o));
     if ((threadIdx.x & 63) == 0) L.red[threadIdx.x >> 6] = zabs;
+    if (threadIdx.x < 64) {
+      const float sc = knot_scale_from_parts(ev);
+      if (threadIdx.x == 0) {
+        L.sc = sc;
+        ev.mscale[FSM_KS_PARTS] = sc;  // the same value from every block
+      }
+    }
   }
   for (int k = threadIdx.x; k < FSMSP_TX * FSM_SPLINE_W; k += blockDim.x) {
     const int gx = x0 + k / FSM_SPLINE_W;
@@ -153,7 +151,7 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
       float zx = L.zxr[lx * SP_RY + ly + B];
       ev.planes[(long)gx * ny + gy] = make_float4(z, zy, zx, zxy);
       if (ev.planes_h) {
-        const float sc = *ev.mscale;
+        const float sc = L.sc;
         const float tol = -FSM_MFMA_POS_EPS * fmaxf(fmaxf(L.red[0], L.red[1]), fmaxf(L.red[2], L.red[3]));
         const float big = fmaxf(fmaxf(fabsf(z), fabsf(zy)), fmaxf(fabsf(zx), fabsf(zxy))) * sc;
         bad |= !(knot_cp_min(z, zy, zx, zxy) >= tol) || !(big <= FSM_MFMA_MAXABS);
@@ -175,40 +173,62 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
 }
 
 // Knot scale of the MFMA form: 2^k <= FSM_MFMA_ZTARGET / max|z| over the
-// event (z = e0 / 2 pi, as the spline kernel computes it).  One workgroup
-// per event, no pre-cleared memory (the kernel may be captured in a graph).
-__device__ void knot_scale_event(const Event2D &ev, float *red) {
+// event (z as the spline kernel computes it).  Pass 1 (k_knot_scale): part p
+// of FSM_KS_PARTS writes the largest |z| of its contiguous chunk to
+// mscale[p]; pass 2 (the spline kernel's blocks) reduce the parts and each
+// writes the same scale to mscale[FSM_KS_PARTS].  No pre-cleared memory and
+// no atomics, so both passes may be captured in a graph.
+__device__ void knot_scale_part(const Event2D &ev, int part, float *red) {
   const long n = (long)ev.nx * ev.ny;
-  float m = 0.0f;
-  for (long k = threadIdx.x; k < n; k += blockDim.x)
-    m = fmaxf(m, fabsf((float)((double)ingest_e0(ev.e_gev[k]) / (2.0 * M_PI))));
+  const long k0 = n * part / FSM_KS_PARTS, k1 = n * (part + 1) / FSM_KS_PARTS;
+  float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+  const long step = blockDim.x;
+  long k = k0 + threadIdx.x;
+  for (; k + 3 * step < k1; k += 4 * step) {
+    m0 = fmaxf(m0, fabsf(knot_z(ev, ev.e_gev[k])));
+    m1 = fmaxf(m1, fabsf(knot_z(ev, ev.e_gev[k + step])));
+    m2 = fmaxf(m2, fabsf(knot_z(ev, ev.e_gev[k + 2 * step])));
+    m3 = fmaxf(m3, fabsf(knot_z(ev, ev.e_gev[k + 3 * step])));
+  }
+  for (; k < k1; k += step) m0 = fmaxf(m0, fabsf(knot_z(ev, ev.e_gev[k])));
+  float m = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
-    float s = 1.0f;
-    if (m > 0.0f && m < INFINITY) {
-      int ex = 0;
-      frexpf(FSM_MFMA_ZTARGET / m, &ex);
-      s = ldexpf(1.0f, min(max(ex - 1, -60), 60));
-    }
-    *ev.mscale = s;
+    ev.mscale[part] = m;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_knot_scale_one(Event2D ev) {
-  __shared__ float red[16];
-  knot_scale_event(ev, red);
+// Scale from the parts (every lane of a wave returns it).
+__device__ float knot_scale_from_parts(const Event2D &ev) {
+  float m = 0.0f;
+  for (int p = threadIdx.x & 63; p < FSM_KS_PARTS; p += 64) m = fmaxf(m, ev.mscale[p]);
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  float s = 1.0f;
+  if (m > 0.0f && m < INFINITY) {
+    int ex = 0;
+    frexpf(FSM_MFMA_ZTARGET / m, &ex);
+    s = ldexpf(1.0f, min(max(ex - 1, -60), 60));
+  }
+  return s;
 }
 
-__global__ __launch_bounds__(1024) void k_knot_scale_batch(const Event2D *__restrict__ evs,
-                                                           const int4 *__restrict__ items, int n_items) {
-  __shared__ float red[16];
+__global__ __launch_bounds__(256) void k_knot_scale_one(Event2D ev) {
+  __shared__ float red[4];
+  knot_scale_part(ev, blockIdx.x, red);
+}
+
+// Batched: item = (event, part).
+__global__ __launch_bounds__(256) void k_knot_scale_batch(const Event2D *__restrict__ evs,
+                                                          const int4 *__restrict__ items, int n_items) {
+  __shared__ float red[4];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const Event2D ev = evs[items[it].x];
-    if (ev.planes_h) knot_scale_event(ev, red);
+    const int4 item = items[xcd_order(it, n_items)];
+    const Event2D ev = evs[item.x];
+    if (ev.planes_h) knot_scale_part(ev, item.y, red);
     __syncthreads();
   }
 }
@@ -238,14 +258,19 @@ __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restric
 // e_gev (reference layout [nxy][neta]) -> src (eta-major [neta][nxy]) with
 // src = e0 * norm_factor (src/FreeStream.cpp:346, first product).
 __device__ void src3d_tile(const float *__restrict__ e_gev, float *__restrict__ src, int nxy, int neta,
-                           float norm_factor, int bx, int by, float (*tile)[65]) {
+                           float norm_factor, int charge, int bx, int by, float (*tile)[65]) {
   const int xy0 = bx * 64, et0 = by * 64;
   const int tx = threadIdx.x & 63, tyb = threadIdx.x >> 6;
   __syncthreads();
   for (int r = tyb; r < 64; r += 4) {
     int xy = xy0 + r, et = et0 + tx;
     float v = 0.0f;
-    if (xy < nxy && et < neta) v = ingest_e0(e_gev[(long)xy * neta + et]) * norm_factor;
+    if (xy < nxy && et < neta) {
+      const float x = e_gev[(long)xy * neta + et];
+      // src/FreeStream.cpp:384 (n_B * norm) or :346 (e0 * norm); the
+      // rapidity factor is applied per slot
+      v = (charge ? x : ingest_e0(x)) * norm_factor;
+    }
     tile[r][tx] = v;
   }
   __syncthreads();
@@ -256,9 +281,9 @@ __device__ void src3d_tile(const float *__restrict__ e_gev, float *__restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_src3d(const float *__restrict__ e_gev, float *__restrict__ src,
-                                               int nxy, int neta, float norm_factor) {
+                                               int nxy, int neta, float norm_factor, int charge) {
   __shared__ float tile[64][65];
-  src3d_tile(e_gev, src, nxy, neta, norm_factor, blockIdx.x, blockIdx.y, tile);
+  src3d_tile(e_gev, src, nxy, neta, norm_factor, charge, blockIdx.x, blockIdx.y, tile);
 }
 
 // Batched: item = (event, xy tile, eta tile).
@@ -268,7 +293,7 @@ __global__ __launch_bounds__(256) void k_src3d_batch(const Event3D *__restrict__
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     int4 item = items[xcd_order(it, n_items)];
     const Event3D ev = evs[item.x];
-    src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, item.y, item.z, tile);
+    src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, ev.charge, item.y, item.z, tile);
   }
 }
 
@@ -366,8 +391,8 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
 #pragma unroll
       for (int m = 0; m < FSM3D_M; ++m) {
         const int sxi = cx[m] + ox, syi = cy[m] + oy;
-        const bool ok = live[m] && mask_ok(cx[m], sxi, dxj, ev.xmin, ev.dx, nx) &&
-                        mask_ok(cy[m], syi, dyj, ev.ymin, ev.dy, ny);
+        const bool ok = live[m] && fsm_mask_ok(cx[m], sxi, dxj, ev.xmin, ev.dx, nx) &&
+                        fsm_mask_ok(cy[m], syi, dyj, ev.ymin, ev.dy, ny);
         float v = 0.0f;
         if (ok) {
           float q00, q01, q10, q11;
@@ -423,8 +448,9 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
 
 __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_one(Event3D ev) {
   extern __shared__ float qlds[];
-  const int t = xcd_order(blockIdx.x, gridDim.x);
-  stream3d_tile(ev, t % ev.tiles, t / ev.tiles, qlds);
+  // round-robin over the XCDs: the work of an eta plane depends on how many
+  // of its rapidity slots are valid, so contiguous ranges would unbalance
+  stream3d_tile(ev, blockIdx.x % ev.tiles, blockIdx.x / ev.tiles, qlds);
 }
 
 __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D *__restrict__ evs,
@@ -432,7 +458,7 @@ __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D 
                                                                   int n_items) {
   extern __shared__ float qlds[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[xcd_order(it, n_items)];
+    int4 item = items[it];  // round-robin over the XCDs (see k_stream3d_one)
     const Event3D ev = evs[item.x];
     stream3d_tile(ev, item.y, item.z, qlds);
   }
@@ -660,6 +686,17 @@ __device__ void landau_cell_out(const EventLandau &ev, long is) {
       for (int c = 0; c < 17; ++c) o[c * P + is] = 0.0;
     }
   }
+  if (ev.J) {
+    // src/LandauMatch.cpp:370-392: n_B = u_mu J^mu, V^mu = J^mu - n_B u^mu
+    float j[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) j[c] = ev.J[c * ev.t_plane + tidx];
+    const float nb = (u[0] * j[0]) - (u[1] * j[1]) - (u[2] * j[2]) - (TAU * TAU * u[3] * j[3]);
+    const long P = ev.out_plane;
+    ev.out_b[is] = (double)nb;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ev.out_b[(1 + c) * P + is] = (double)(j[c] - (nb * u[c]));
+  }
   const long D = ev.dim;
   if (ev.fe) ev.fe[is] = e;
   if (ev.fu) {
@@ -697,7 +734,7 @@ __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restr
 // ======================================================================
 hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s) {
   if (ev.planes_h) {
-    hipLaunchKernelGGL(k_knot_scale_one, dim3(1), dim3(1024), 0, s, ev);
+    hipLaunchKernelGGL(k_knot_scale_one, dim3(FSM_KS_PARTS), dim3(256), 0, s, ev);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -709,7 +746,7 @@ hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s) {
 hipError_t fsm_launch_src3d(const Event3D &ev, hipStream_t s) {
   int nxy = ev.nx * ev.ny;
   dim3 grid((nxy + 63) / 64, (ev.neta + 63) / 64);
-  hipLaunchKernelGGL(k_src3d, grid, dim3(256), 0, s, ev.e_gev, ev.src, nxy, ev.neta, ev.norm_factor);
+  hipLaunchKernelGGL(k_src3d, grid, dim3(256), 0, s, ev.e_gev, ev.src, nxy, ev.neta, ev.norm_factor, ev.charge);
   return hipGetLastError();
 }
 
@@ -747,7 +784,7 @@ static inline int batch_grid(int n_items) { return std::max(1, std::min(n_items,
 
 hipError_t fsm_launch_knot_scale_batch(const Event2D *evs, const int4 *items, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_knot_scale_batch, dim3(batch_grid(n)), dim3(1024), 0, s, evs, items, n);
+  hipLaunchKernelGGL(k_knot_scale_batch, dim3(batch_grid(n)), dim3(256), 0, s, evs, items, n);
   return hipGetLastError();
 }
 

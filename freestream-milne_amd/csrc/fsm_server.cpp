@@ -91,6 +91,7 @@ struct Request {
   int64_t id = 0;
   fsm_params p{};
   size_t in_bytes = 0, out_bytes = 0, ws_bytes = 0, footprint = 0;
+  bool baryon = false;   // input e then n_B; output FSM_NUM_OUTPUTS_BARYON planes
   float *d_in = nullptr;
   double *d_out = nullptr;
   void *ws = nullptr;
@@ -196,7 +197,8 @@ void worker_loop(fsm_server *s) {
         for (Request *r : take) {
           if (st) break;
           if (hipStreamWaitEvent(sl.stream, r->uploaded, 0) != hipSuccess) st = FSM_ERR_DEVICE;
-          int i = fsm_host::batch_add_ws(sl.batch, &r->p, r->d_in, r->d_out, r->ws);
+          int i = fsm_host::batch_add_ws(sl.batch, &r->p, r->d_in, r->baryon ? r->d_in + r->p.DIM : nullptr,
+                                         r->d_out, r->ws);
           if (i < 0) st = i;
         }
         if (!st) st = fsm_host::batch_launch_now(sl.batch, sl.stream);
@@ -293,15 +295,18 @@ int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV
   fsm_params q = *p;
   int st = fsm_params_finalize(&q, e_GeV, n);
   if (st) return st;
-  if (q.BARYON || nB) return FSM_ERR_UNSUPPORTED;
+  // BARYON = 1 needs n_B; with BARYON = 0 an n_B is ignored, as the reference ignores charge then
+  if (q.BARYON && !nB) return FSM_ERR_ARG;
+  const bool baryon = q.BARYON && nB;
   st = fsm_host::plan_check(s->ctx, &q);
   if (st) return st;
   Request *r = new (std::nothrow) Request();
   if (!r) return FSM_ERR_OOM;
   r->p = q;
-  r->in_bytes = (size_t)q.DIM * sizeof(float);
-  r->out_bytes = (size_t)q.DIM * FSM_NUM_OUTPUTS * sizeof(double);
-  r->ws_bytes = fsm_host::workspace_bytes(q);
+  r->baryon = baryon;
+  r->in_bytes = (size_t)q.DIM * sizeof(float) * (baryon ? 2 : 1);
+  r->out_bytes = (size_t)q.DIM * (baryon ? FSM_NUM_OUTPUTS_BARYON : FSM_NUM_OUTPUTS) * sizeof(double);
+  r->ws_bytes = fsm_host::workspace_bytes(q, baryon);
   r->footprint = r->in_bytes + r->out_bytes + r->ws_bytes;
   {
     std::lock_guard<std::mutex> g(s->mu);
@@ -328,7 +333,8 @@ int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV
   }
   if (ok && !r->uploaded) ok = hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming) == hipSuccess;
   if (ok) {
-    memcpy(r->h_in, e_GeV, r->in_bytes);
+    memcpy(r->h_in, e_GeV, (size_t)q.DIM * sizeof(float));
+    if (baryon) memcpy(r->h_in + q.DIM, nB, (size_t)q.DIM * sizeof(float));
     ok = hipMemcpyAsync(r->d_in, r->h_in, r->in_bytes, hipMemcpyHostToDevice, s->upload_stream) == hipSuccess &&
          hipEventRecord(r->uploaded, s->upload_stream) == hipSuccess;
   }
@@ -398,6 +404,38 @@ int fsm_server_fetch(fsm_server *s, int64_t id, double *const out[FSM_NUM_OUTPUT
     for (int k = 0; k < FSM_NUM_OUTPUTS && st == FSM_OK; ++k)
       if (out[k] && hipMemcpy(out[k], d_out + k * D, D * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         st = FSM_ERR_DEVICE;
+  }
+  int rs = fsm_server_release(s, id);
+  return st ? st : rs;
+}
+
+int fsm_server_fetch_baryon(fsm_server *s, int64_t id, double *const out[FSM_NUM_OUTPUTS], double *nB_out,
+                            double *const V_out[4]) {
+  if (!s || !out) return FSM_ERR_ARG;
+  double *d_out = nullptr;
+  int st = fsm_server_result(s, id, nullptr, &d_out);
+  if (st == FSM_ERR_BUSY || st == FSM_ERR_ARG) return st;
+  if (st == FSM_OK) {
+    size_t D;
+    bool baryon;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      D = (size_t)s->reqs[id]->p.DIM;
+      baryon = s->reqs[id]->baryon;
+    }
+    (void)hipSetDevice(fsm_host::ctx_device(s->ctx));
+    auto get = [&](double *dst, int plane) {
+      if (dst && st == FSM_OK && hipMemcpy(dst, d_out + plane * D, D * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        st = FSM_ERR_DEVICE;
+    };
+    for (int k = 0; k < FSM_NUM_OUTPUTS; ++k) get(out[k], k);
+    if (baryon) {
+      get(nB_out, FSM_NUM_OUTPUTS);
+      if (V_out)
+        for (int c = 0; c < 4; ++c) get(V_out[c], FSM_NUM_OUTPUTS + 1 + c);
+    } else if (nB_out || V_out) {
+      st = FSM_ERR_ARG;  // not a baryon request
+    }
   }
   int rs = fsm_server_release(s, id);
   return st ? st : rs;

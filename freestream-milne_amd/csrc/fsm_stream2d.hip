@@ -28,19 +28,6 @@
 
 namespace {
 
-__device__ __forceinline__ bool ref_mask_ok2(int i, float shift, float mn, float h, int n) {
-  const float x = __fadd_rn(__fmul_rn((float)i, h), mn);
-  const float xn = __fsub_rn(x, shift);
-  const float f = floorf(__fdiv_rn(__fsub_rn(xn, mn), h));
-  return (f >= 1.0f) && (f < (float)(n - 1));
-}
-
-// src/FreeStream.cpp:213 mask; exact float re-evaluation next to the boundary.
-__device__ __forceinline__ bool mask_ok2(int i, int g, float shift, float mn, float h, int n) {
-  if ((unsigned)g <= 1u || (unsigned)(g - (n - 2)) <= 1u) return ref_mask_ok2(i, shift, mn, h, n);
-  return (unsigned)(g - 1) <= (unsigned)(n - 3);
-}
-
 // Copies the knot box [bx, bx+LX) x [by, by+LY) of an (nx, ny) x-major plane
 // into LDS (zero outside the grid), column pitch `pitch` >= LY elements.  Each
 // thread keeps U loads in flight and walks the box with an incremental (row,
@@ -171,10 +158,10 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
         }
       }
       if (!interior) {
-        const bool yok = mask_ok2(iy, iy + oy, S.dyj, ev.ymin, ev.dy, ny);
+        const bool yok = fsm_mask_ok(iy, iy + oy, S.dyj, ev.ymin, ev.dy, ny);
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          const bool ok = yok && mask_ok2(ixb + m, ixb + m + ox, S.dxj, ev.xmin, ev.dx, nx);
+          const bool ok = yok && fsm_mask_ok(ixb + m, ixb + m + ox, S.dxj, ev.xmin, ev.dx, nx);
           v[m] = ok ? v[m] : 0.0f;
         }
       }
@@ -215,7 +202,13 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int ix = ixb + m;
-    if (ix < nx && iy < ny) {
+    if (ix < nx && iy < ny && ev.out_j) {
+      // src/LandauMatch.cpp:140: J = sum * DRAP * d_phip; rows 1, c, s, tanh(0) = 0
+      const long is = (long)ix * ny + iy;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ev.T[c * P + is] = ((acc[m][c] * ev.out_scale) * ev.jscale) * ev.d_phip;
+      ev.T[3 * P + is] = 0.0f;
+    } else if (ix < nx && iy < ny) {
       const long is = (long)ix * ny + iy;
       float r[6];
       const float s5 = acc[m][0] - acc[m][3];
@@ -267,7 +260,7 @@ __device__ __forceinline__ float4 plane_or_zero(const Event2D &ev, int gx, int g
 // rounding of an integer the reference's floor can be ix + ox -+ 1; away from
 // the mask boundary that does not change the decision (and the value is
 // continuous across the knot line), but for a floor index in {0, 1, n-2, n-1}
-// the reference's own float expression decides (mask_ok2).  Returns the exact
+// the reference's own float expression decides (fsm_mask_ok).  Returns the exact
 // sample (bilinear where the bicubic is not > 0, src/FreeStream.cpp:245-246)
 // minus what the MFMA form added.
 __device__ __forceinline__ float sample_correction(const Event2D &ev, const Stencil2D &S, int ix, int iy) {
@@ -276,7 +269,7 @@ __device__ __forceinline__ float sample_correction(const Event2D &ev, const Sten
   const bool by = (unsigned)gy <= 1u || (unsigned)(gy - (ev.ny - 2)) <= 1u;
   if (!bx && !by) return 0.0f;
   const bool idx = (unsigned)(gx - 1) <= (unsigned)(ev.nx - 3) && (unsigned)(gy - 1) <= (unsigned)(ev.ny - 3);
-  const bool ok = mask_ok2(iy, gy, S.dyj, ev.ymin, ev.dy, ev.ny) && mask_ok2(ix, gx, S.dxj, ev.xmin, ev.dx, ev.nx);
+  const bool ok = fsm_mask_ok(iy, gy, S.dyj, ev.ymin, ev.dy, ev.ny) && fsm_mask_ok(ix, gx, S.dxj, ev.xmin, ev.dx, ev.nx);
   if (!ok && !idx) return 0.0f;
   const float4 k00 = plane_or_zero(ev, gx, gy), k01 = plane_or_zero(ev, gx, gy + 1);
   const float4 k10 = plane_or_zero(ev, gx + 1, gy), k11 = plane_or_zero(ev, gx + 1, gy + 1);
@@ -301,21 +294,29 @@ __device__ __forceinline__ float sample_correction(const Event2D &ev, const Sten
 // with weights B' (corner-x roles swapped: B'[lane] = B[lane ^ 16], a lane
 // swizzle).  So a wave loads XW/2 + 1 knot-column pairs per row block instead
 // of XW.
-// MASKED: src/FreeStream.cpp:213 mask, exact here (the floor index of an MFMA
-// sample is ix + ox for every cell): per offset a cell column is in or out as
-// a whole, a 16-row block is all in, all out, or split (per-lane select);
-// blocks with no sample in range issue no MFMA.
+// MASKED: edge mask (src/FreeStream.cpp:213), exact here since the floor
+// index of an MFMA sample is ix + ox for every cell.  Rows out of range zero
+// their knot registers (a 16-row block is all in, all out, or split), columns
+// out of range skip their MFMAs; both tests are wave-uniform.
 template <int XW, bool MASKED>
 __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
                                              const int4 *orec, int no, int lane_base, int LYP, int gx, int r,
                                              int xw0, int y0, int nx, int ny) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
+  // the offset record and weights of the next offset are read one iteration
+  // ahead, off the critical path of the knot loads
+  int4 rec_n = orec[0];
+  uint4 bw_n = bl[lane];
   for (int o = 0; o < no; ++o) {
-    const int4 rec = orec[o];
+    const int4 rec = rec_n;
+    const uint4 bw = bw_n;
+    if (o + 1 < no) {
+      rec_n = orec[o + 1];
+      bw_n = bl[(o + 1) * 64 + lane];
+    }
     const int ox = __builtin_amdgcn_readfirstlane(rec.x), oy = __builtin_amdgcn_readfirstlane(rec.y);
     const int delta = __builtin_amdgcn_readfirstlane(rec.z);
-    const uint4 bw = bl[o * 64 + lane];
     const fsm_half8 b0 = __builtin_bit_cast(fsm_half8, bw);
     // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 -> lane ^ 16 within 32 lanes
     const uint4 bs = make_uint4(__builtin_amdgcn_ds_swizzle((int)bw.x, 0x401F), __builtin_amdgcn_ds_swizzle((int)bw.y, 0x401F),
@@ -333,12 +334,19 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
       }
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
-      bool yfull = true, yok = true;
       if (MASKED) {
         const int ylo = y0 + yb * 16 + oy;  // floor index of row r is ylo + r
         if (ylo + 15 < 1 || ylo > ny - 2) continue;
-        yfull = ylo >= 1 && ylo + 15 <= ny - 2;
-        yok = (unsigned)(ylo + r - 1) <= (unsigned)(ny - 3);
+        if (!(ylo >= 1 && ylo + 15 <= ny - 2)) {
+          const bool yok = (unsigned)(ylo + r - 1) <= (unsigned)(ny - 3);
+#pragma unroll
+          for (int k = 0; k < NR; ++k) {
+            R[k][yb].x = yok ? R[k][yb].x : 0u;
+            R[k][yb].y = yok ? R[k][yb].y : 0u;
+            R[k][yb].z = yok ? R[k][yb].z : 0u;
+            R[k][yb].w = yok ? R[k][yb].w : 0u;
+          }
+        }
       }
 #pragma unroll
       for (int xc = 0; xc < XW; ++xc) {
@@ -351,17 +359,93 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
           a.z = gx ? a.z : n1.z;
           a.w = gx ? a.w : n1.w;
         }
-        if (MASKED && !yfull) {
-          a.x = yok ? a.x : 0u;
-          a.y = yok ? a.y : 0u;
-          a.z = yok ? a.z : 0u;
-          a.w = yok ? a.w : 0u;
-        }
         acc[xc][yb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(fsm_half8, a), (xc & 1) ? b1 : b0,
                                                              acc[xc][yb], 0, 0, 0);
       }
     }
   }
+}
+
+// Registers that carry one offset group's LDS image -- knot box, weights,
+// offset records -- from its global loads (issued before the previous
+// group's MFMAs, so they overlap them) to its LDS stores.  Elements past the
+// register capacity are copied directly at store time.
+constexpr int kPfBox = 8, kPfW = 2;
+struct GroupPrefetch {
+  uint4 box[kPfBox];
+  uint4 w[kPfW];
+  int4 rec;
+};
+
+// Walks box element k = threadIdx.x + u * blockDim.x as (column, row).
+struct BoxWalk {
+  int lx, ly, dlx, dly, LY;
+  __device__ __forceinline__ BoxWalk(int LY_) : LY(LY_) {
+    dlx = blockDim.x / LY;
+    dly = blockDim.x - dlx * LY;
+    lx = threadIdx.x / LY;
+    ly = threadIdx.x - lx * LY;
+  }
+  __device__ __forceinline__ void next() {
+    lx += dlx;
+    ly += dly;
+    if (ly >= LY) {
+      ly -= LY;
+      ++lx;
+    }
+  }
+};
+
+__device__ __forceinline__ void group_load(GroupPrefetch &pf, const Event2D &ev, const StencilGroup &G, int x0,
+                                           int y0, int TX, int TY) {
+  const int nx = ev.nx, ny = ev.ny;
+  const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0;
+  const int n = LX * LY, bx = x0 + G.bx0, by = y0 + G.by0;
+  BoxWalk wk(LY);
+#pragma unroll
+  for (int u = 0; u < kPfBox; ++u) {
+    const int k = threadIdx.x + u * blockDim.x;
+    const int kx = bx + wk.lx, ky = by + wk.ly;
+    pf.box[u] = make_uint4(0u, 0u, 0u, 0u);
+    if (k < n && (unsigned)kx < (unsigned)nx && (unsigned)ky < (unsigned)ny) pf.box[u] = ev.planes_h[(long)kx * ny + ky];
+    wk.next();
+  }
+  const int nw = (G.j1 - G.j0) * 64;
+#pragma unroll
+  for (int u = 0; u < kPfW; ++u) {
+    const int k = threadIdx.x + u * blockDim.x;
+    if (k < nw) pf.w[u] = ev.bmat[(long)G.j0 * 64 + k];
+  }
+  if ((int)threadIdx.x < G.j1 - G.j0) pf.rec = ev.moffs[G.j0 + threadIdx.x];
+}
+
+__device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event2D &ev, const StencilGroup &G,
+                                            uint4 *lh, int x0, int y0, int TX, int TY) {
+  const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
+  const int no = G.j1 - G.j0, n = LX * LY, step = blockDim.x;
+  uint4 *bl = lh + LX * LYP;
+  int4 *orec = reinterpret_cast<int4 *>(bl + no * 64);
+  BoxWalk wk(LY);
+#pragma unroll
+  for (int u = 0; u < kPfBox; ++u) {
+    if ((int)threadIdx.x + u * step < n) lh[wk.lx * LYP + wk.ly] = pf.box[u];
+    wk.next();
+  }
+  for (int k = threadIdx.x + kPfBox * step; k < n; k += step) {  // beyond the registers
+    const int lx = k / LY, ly = k - lx * LY;
+    const int kx = x0 + G.bx0 + lx, ky = y0 + G.by0 + ly;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if ((unsigned)kx < (unsigned)ev.nx && (unsigned)ky < (unsigned)ev.ny) v = ev.planes_h[(long)kx * ev.ny + ky];
+    lh[lx * LYP + ly] = v;
+  }
+  const int nw = no * 64;
+#pragma unroll
+  for (int u = 0; u < kPfW; ++u) {
+    const int k = threadIdx.x + u * step;
+    if (k < nw) bl[k] = pf.w[u];
+  }
+  for (int k = threadIdx.x + kPfW * step; k < nw; k += step) bl[k] = ev.bmat[(long)G.j0 * 64 + k];
+  if ((int)threadIdx.x < no) orec[threadIdx.x] = pf.rec;
 }
 
 // MFMA tile shapes and the FP32 tile of the same width they fall back to.
@@ -376,7 +460,8 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
   constexpr int TX = 4 * XW, TY = 32, NC = TX * TY;
   const int nx = ev.nx, ny = ev.ny;
   const int x0 = tile_x * TX, y0 = tile_y * TY;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   {
     // spline tiles holding the knots this tile can read
     const int sx0 = max(0, x0 + ev.rbx0) / FSMSP_TX, sx1 = min(nx - 1, x0 + TX - 1 + ev.rbx1) / FSMSP_TX;
@@ -396,7 +481,9 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
   for (int xc = 0; xc < XW; ++xc)
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) acc[xc][yb] = fsm_float4v{0.f, 0.f, 0.f, 0.f};
-  const int xlast = min(x0 + TX - 1, nx - 1), ylast = min(y0 + TY - 1, ny - 1);
+  const int ylast = min(y0 + TY - 1, ny - 1);
+  GroupPrefetch pf;
+  if (ev.n_mgroups > 0) group_load(pf, ev, ev.mgroups[0], x0, y0, TX, TY);
   for (int gi = 0; gi < ev.n_mgroups; ++gi) {
     const StencilGroup G = ev.mgroups[gi];
     // box columns are padded to 16 records (256 B) so that the lanes reading
@@ -406,19 +493,21 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     uint4 *bl = lh + LX * LYP;  // [no][64] weights
     int4 *orec = reinterpret_cast<int4 *>(bl + no * 64);  // [no] (ox, oy, box delta, -)
     __syncthreads();
-    stage_box<8>(lh, ev.planes_h, x0 + G.bx0, y0 + G.by0, LX, LY, nx, ny, LYP);
-    stage_copy<4>(bl, ev.bmat + (long)G.j0 * 64, no * 64);
-    stage_copy<1>(orec, ev.moffs + G.j0, no);
+    group_store(pf, ev, G, lh, x0, y0, TX, TY);
     __syncthreads();
+    // next group's loads fly while this group's MFMAs run
+    if (gi + 1 < ev.n_mgroups) group_load(pf, ev, ev.mgroups[gi + 1], x0, y0, TX, TY);
     // every real cell's floor index stays inside [1, n-2] for all offsets of the group
     // lane's knot (corner) for cell column wave*XW, row r, at box offset 0
     const int lane_base = (wave * XW + gx) * LYP + r + gy;
-    const bool interior = (x0 + G.bx0 >= 1) && (xlast + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
+    // this wave's cells keep every floor index of the group inside [1, n-2]
+    const int xw0 = x0 + wave * XW, xw1 = min(xw0 + XW - 1, nx - 1);
+    const bool interior = (xw0 + G.bx0 >= 1) && (xw1 + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
                           (ylast + G.by1 - 1 <= ny - 2);
     if (interior)
-      mfma_offsets<XW, false>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, 0, 0, 0, 0);
+      mfma_offsets<XW, false>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
     else
-      mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, x0 + wave * XW, y0, nx, ny);
+      mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
   }
   // moments to LDS: Tt[m][lx*TY + ly]
   __syncthreads();
@@ -439,7 +528,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
   __syncthreads();
   // src/LandauMatch.cpp:104: T = sum * d_phip / TAU; rows 3, 6, 8, 9 carry tanh(0) = 0
   const long P = ev.t_plane;
-  const float inv_sc = 1.0f / *ev.mscale;  // a power of two: exact
+  const float inv_sc = 1.0f / ev.mscale[FSM_KS_PARTS];  // a power of two: exact
   constexpr int CPT = (NC + 255) / 256;     // cells per thread
   float t[CPT][6];
   int cix[CPT], ciy[CPT];
@@ -473,6 +562,14 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
     if (!live[q]) continue;
+    if (ev.out_j) {
+      // src/LandauMatch.cpp:140: J = sum * DRAP * d_phip; rows 1, c, s, tanh(0) = 0
+      const long is = (long)cix[q] * ny + ciy[q];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ev.T[c * P + is] = ((t[q][c] * ev.out_scale) * ev.jscale) * ev.d_phip;
+      ev.T[3 * P + is] = 0.0f;
+      continue;
+    }
     float rr[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) rr[m] = ((t[q][m] * ev.out_scale) * ev.d_phip) / ev.tau;

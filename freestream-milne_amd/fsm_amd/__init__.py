@@ -32,6 +32,7 @@ OUTPUT_NAMES = (
     "Pi",
 )
 NUM_OUTPUTS = len(OUTPUT_NAMES)
+NUM_OUTPUTS_BARYON = NUM_OUTPUTS + 5  # + n_B, V^tau, V^x, V^y, V^eta
 
 FSM_OK = 0
 FSM_ERR_SIZE = -1
@@ -146,6 +147,7 @@ def _lib():
         lib.fsm_run_baryon.argtypes = [vp, P, fp, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), fp]
         lib.fsm_run_device.argtypes = [vp, P, vp, vp, vp]
+        lib.fsm_run_baryon_device.argtypes = [vp, P, vp, vp, vp, vp]
         lib.fsm_stream_tmunu_device.argtypes = [vp, P, vp, vp, vp]
         lib.fsm_landau_device.argtypes = [vp, P, vp, vp, vp, vp, vp, vp, vp]
         lib.fsm_sync.argtypes = [vp, vp]
@@ -156,6 +158,7 @@ def _lib():
         lib.fsm_batch_destroy.argtypes = [vp]
         lib.fsm_batch_destroy.restype = None
         lib.fsm_batch_add.argtypes = [vp, P, vp, vp]
+        lib.fsm_batch_add_baryon.argtypes = [vp, P, vp, vp, vp]
         lib.fsm_batch_clear.argtypes = [vp]
         lib.fsm_batch_size.argtypes = [vp]
         lib.fsm_batch_set_graph.argtypes = [vp, ctypes.c_int]
@@ -170,6 +173,8 @@ def _lib():
         lib.fsm_server_result.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_void_p)]
         lib.fsm_server_fetch.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]
+        lib.fsm_server_fetch_baryon.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                                ctypes.POINTER(ctypes.c_void_p)]
         lib.fsm_server_release.argtypes = [vp, ctypes.c_int64]
         lib.fsm_server_cancel.argtypes = [vp, ctypes.c_int64]
         lib.fsm_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
@@ -292,6 +297,11 @@ class Context:
         _check(_lib().fsm_run_device(self._ctx, ctypes.byref(p), d_e, d_out, _stream_handle(stream)),
                "fsm_run_device")
 
+    def run_baryon_device(self, p: Params, d_e: int, d_nB: int, d_out: int, stream=None):
+        """d_out: NUM_OUTPUTS_BARYON planes of DIM doubles (17 outputs, n_B, V^mu)."""
+        _check(_lib().fsm_run_baryon_device(self._ctx, ctypes.byref(p), d_e, d_nB, d_out, _stream_handle(stream)),
+               "fsm_run_baryon_device")
+
     def stream_tmunu_device(self, p: Params, d_e: int, d_T: int, stream=None):
         _check(_lib().fsm_stream_tmunu_device(self._ctx, ctypes.byref(p), d_e, d_T, _stream_handle(stream)),
                "fsm_stream_tmunu_device")
@@ -359,6 +369,15 @@ class Batch:
         self._params.append(q)
         return st
 
+    def add_baryon(self, p: Params, d_e: int, d_nB: int, d_out: int) -> int:
+        """A BARYON = 1 event; d_out holds NUM_OUTPUTS_BARYON planes."""
+        q = p.copy()
+        st = _lib().fsm_batch_add_baryon(self._b, ctypes.byref(q), d_e, d_nB, d_out)
+        if st < 0:
+            raise FsmError(st, "fsm_batch_add_baryon")
+        self._params.append(q)
+        return st
+
     def clear(self):
         _check(_lib().fsm_batch_clear(self._b), "fsm_batch_clear")
         self._params = []
@@ -402,10 +421,12 @@ class Server:
         _check(_lib().fsm_server_create(ctx._ctx, ctypes.byref(cfg), ctypes.byref(self._s)), "fsm_server_create")
         self._dims = {}
 
-    def submit(self, p: Params, e: np.ndarray):
+    def submit(self, p: Params, e: np.ndarray, nB: np.ndarray = None):
+        """nB (initial n_B, fm^-3) for a BARYON = 1 request."""
         e = np.ascontiguousarray(e, dtype=np.float32).ravel()
+        nb = None if nB is None else np.ascontiguousarray(nB, dtype=np.float32).ravel()
         q = p.copy()
-        rid = _lib().fsm_server_submit(self._s, ctypes.byref(q), _fptr(e), None, e.size)
+        rid = _lib().fsm_server_submit(self._s, ctypes.byref(q), _fptr(e), None if nb is None else _fptr(nb), e.size)
         if rid == FSM_ERR_BUSY:
             return None
         if rid < 0:
@@ -429,6 +450,17 @@ class Server:
         arr = (ctypes.c_void_p * NUM_OUTPUTS)(*[o.ctypes.data for o in outs])
         _check(_lib().fsm_server_fetch(self._s, rid, arr), "fsm_server_fetch")
         return dict(zip(OUTPUT_NAMES, outs))
+
+    def fetch_baryon(self, rid: int):
+        """(outputs dict, n_B, [V^tau, V^x, V^y, V^eta]) of a baryon request."""
+        n = self._dims.pop(rid)
+        outs = [np.empty(n, dtype=np.float64) for _ in range(NUM_OUTPUTS)]
+        nb = np.empty(n, dtype=np.float64)
+        V = [np.empty(n, dtype=np.float64) for _ in range(4)]
+        arr = (ctypes.c_void_p * NUM_OUTPUTS)(*[o.ctypes.data for o in outs])
+        varr = (ctypes.c_void_p * 4)(*[v.ctypes.data for v in V])
+        _check(_lib().fsm_server_fetch_baryon(self._s, rid, arr, nb.ctypes.data, varr), "fsm_server_fetch_baryon")
+        return dict(zip(OUTPUT_NAMES, outs)), nb, V
 
     def release(self, rid: int):
         self._dims.pop(rid, None)
@@ -511,7 +543,8 @@ def exported_symbols_from_header(header_path: str) -> list[str]:
 
 
 __all__ = [
-    "Params", "Context", "FreestreamMilne", "FsmError", "OUTPUT_NAMES", "NUM_OUTPUTS", "params_default",
+    "Params", "Context", "FreestreamMilne", "FsmError", "OUTPUT_NAMES", "NUM_OUTPUTS", "NUM_OUTPUTS_BARYON",
+    "params_default",
     "params_read", "params_finalize", "lib", "version", "LIB_PATH", "PATH_FP32", "PATH_MIXED", "TILES_AUTO",
     "TILES_BATCH",
 ]

@@ -131,3 +131,18 @@ def batch_mix(n_events: int, seed: int = 7):
         e = hotspots_2d(n, n, d, d, ns, seed * 100003 + k)
         events.append((p, e.ravel()))
     return events
+
+
+def baryon_density(e: np.ndarray, scale: float = 0.1) -> np.ndarray:
+    """Initial n_B (fm^-3) as `scale` times the normalised energy profile (the
+    baryon-carrying events of the serving mix)."""
+    e = np.asarray(e, dtype=np.float64)
+    return (scale * e / max(float(e.max()), 1e-30)).astype(np.float32)
+
+
+def with_baryon(p: Params, sigma_b: float = 0.8) -> Params:
+    """Copy of p with BARYON = 1 and SIGMA_B = sigma_b."""
+    q = p.copy()
+    q.BARYON = 1
+    q.SIGMA_B = sigma_b
+    return q

@@ -85,6 +85,9 @@ enum {
  * e, p, ut, ux, uy, un, pitt, pitx, pity, pitn, pixx, pixy, pixn, piyy,
  * piyn, pinn, Pi  (src/FreestreamMilne.cpp:51-67, :700-716). */
 #define FSM_NUM_OUTPUTS 17
+/* Device output of a baryon event: the 17 planes, then n_B (fm^-3) and
+ * V^tau, V^x, V^y, V^eta -- 22 planes of DIM doubles. */
+#define FSM_NUM_OUTPUTS_BARYON 22
 
 /* Library version string. */
 const char *fsm_version(void);
@@ -195,6 +198,12 @@ int fsm_run_baryon(fsm_ctx *ctx, fsm_params *p, const float *e_GeV,
                    const float *nB_in, size_t n,
                    double *const out[FSM_NUM_OUTPUTS], double *nB_out,
                    double *const V_out[4], float *tau_landau_match);
+/* Device-resident form (p finalized, BARYON = 1): d_out holds
+ * FSM_NUM_OUTPUTS_BARYON planes of DIM doubles.  In 2-D, DIM_RAP must be 1
+ * (FSM_ERR_UNSUPPORTED otherwise).  With BARYON = 0 or d_nB = NULL this is
+ * fsm_run_device. */
+int fsm_run_baryon_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, const float *d_nB,
+                          double *d_out, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device-resident entry points.  All pointers are device pointers; `stream`
@@ -239,6 +248,10 @@ typedef struct fsm_batch fsm_batch;
 int fsm_batch_create(fsm_ctx *ctx, fsm_batch **out);
 void fsm_batch_destroy(fsm_batch *b);
 int fsm_batch_add(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, double *d_out);
+/* A baryon event (p->BARYON = 1): d_nB is its initial n_B and d_out holds
+ * FSM_NUM_OUTPUTS_BARYON planes (see fsm_run_baryon_device). */
+int fsm_batch_add_baryon(fsm_batch *b, const fsm_params *p, const float *d_e_GeV, const float *d_nB,
+                         double *d_out);
 int fsm_batch_clear(fsm_batch *b);
 int fsm_batch_size(const fsm_batch *b);
 /* use_graph: 1 (default) capture and replay a HIP graph; 0 launch directly
@@ -277,12 +290,17 @@ int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV
  * within timeout_us microseconds (0: do not wait). */
 int64_t fsm_server_poll(fsm_server *s, int timeout_us);
 /* Completed request: status of its run, submit-to-completion latency in
- * microseconds (may be NULL) and its device output (17*DIM doubles, may be
+ * microseconds (may be NULL) and its device output (17*DIM doubles, 22*DIM
+ * for a baryon request, may be
  * NULL), valid until released. */
 int fsm_server_result(fsm_server *s, int64_t id, double *latency_us, double **d_out);
 /* Copies a completed request's 17 output vectors (DIM doubles each) to host
  * arrays, then releases it. */
 int fsm_server_fetch(fsm_server *s, int64_t id, double *const out[FSM_NUM_OUTPUTS]);
+/* Same for a baryon request (its device output has FSM_NUM_OUTPUTS_BARYON
+ * planes): also n_B and V^mu (either may be NULL). */
+int fsm_server_fetch_baryon(fsm_server *s, int64_t id, double *const out[FSM_NUM_OUTPUTS], double *nB_out,
+                            double *const V_out[4]);
 /* Releases a completed request (its HBM returns to the budget). */
 int fsm_server_release(fsm_server *s, int64_t id);
 /* Cancels a request that has not been launched: FSM_OK (its result status is

@@ -511,7 +511,9 @@ struct Pipeline {
     const int DIM = P.DIM, DIM_ETA = P.DIM_ETA, DIM_Y = P.DIM_Y, DIM_RAP = P.DIM_RAP;
     const R SIGMA_B = (R)P.SIGMA_B, DRAP = (R)P.DRAP, DETA = (R)P.DETA;
     dens.assign((size_t)DIM * DIM_RAP, (R)0);
-    R n = (R)(sqrt(2.0 * M_PI) * (double)SIGMA_B * exp((double)SIGMA_B * (double)SIGMA_B / 2.0));
+    // SIGMA_B * SIGMA_B is a float product in the reference (float operands)
+    const R sb2 = SIGMA_B * SIGMA_B;
+    R n = (R)(sqrt(2.0 * M_PI) * (double)SIGMA_B * exp((double)sb2 / 2.0));
     R norm_factor = (R)(1.0 / (2.0 * M_PI * (double)n));
     R rapmin = (R)((-1.0) * ((double)(R)(DIM_RAP - 1) / 2.0) * (double)DRAP);
     R etamin = (R)((-1.0) * ((double)(R)(DIM_ETA - 1) / 2.0) * (double)DETA);
@@ -523,8 +525,7 @@ struct Pipeline {
       for (int irap = 0; irap < DIM_RAP; ++irap) {
         R rap = (R)irap * DRAP + rapmin;
         R d = eta - rap;
-        R rap_factor = (R)((double)std::cosh(d) *
-                           exp((-1.0) * (double)d * (double)d / ((double)SIGMA_B * (double)SIGMA_B)));
+        R rap_factor = (R)((double)std::cosh(d) * exp((-1.0) * (double)d * (double)d / (double)sb2));
         dens[(size_t)is * DIM_RAP + irap] = n0[is] * norm_factor * rap_factor;
       }
     }

@@ -49,6 +49,16 @@ def interior_mask(q, margin: int = 2):
     return m.ravel()
 
 
+def ring_interior_mask(q, margin: int = 2):
+    """2-D: cells whose every streaming sample lies `margin` or more cells
+    inside the grid (the ring of radius DTAU/DX around the cell).  Float and
+    FP64 keep different samples only at the mask boundary, so this is where
+    the FP64 comparison is meaningful; with DTAU/DX a whole number the
+    boundary samples reach exactly DTAU/DX cells in from the edges."""
+    r = int(np.ceil(float(q.DTAU) / min(float(q.DX), float(q.DY))))
+    return interior_mask(q, margin + r)
+
+
 def well_conditioned(T_ref, tau):
     import oracle as O
 

@@ -23,6 +23,9 @@ def _case(name):
     if name == "rect2d":  # DIM_X != DIM_Y, DX != DY, two identical rapidity copies
         p = W.base_params(DIM_X=77, DIM_Y=64, DIM_RAP=2, DIM_PHIP=40, DX=0.08, DY=0.11, TAU0=0.2, DTAU=0.7)
         return p, W.hotspots_2d(77, 64, 0.08, 0.11, 30, 11).ravel()
+    if name == "int2d":  # DTAU/DX = 10: samples on grid lines next to the mask edge; hotspots at the edges
+        p = W.base_params(DIM_X=101, DIM_Y=101, DIM_PHIP=64, DX=0.08, DY=0.08, TAU0=0.1, DTAU=0.8)
+        return p, W.hotspots_2d(101, 101, 0.08, 0.08, 60, 13).ravel()
     if name in ("spiky", "hotspots_spikes"):
         from test_gpu_mixed import _case as mixed_case
 
@@ -44,7 +47,7 @@ def _gpu(ctx, q, e):
     return d_T.cpu().numpy().reshape(10, D), d_out.cpu().numpy().reshape(17, D)
 
 
-@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "spiky", "hotspots_spikes", "small3d", "small3d_int"])
+@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d", "small3d_int"])
 def test_parity_against_reference_float_path(gpu_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)

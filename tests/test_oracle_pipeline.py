@@ -166,3 +166,57 @@ def test_float_path_close_to_fp64():
     keep = keep.ravel()
     err = np.abs(rf["T"] - rd["T"])[:, keep].max() / np.abs(rd["T"][0]).max()
     assert err < 2e-6
+
+
+# ---------------------------------------------------------------- baryon ----
+def _baryon_params(**kw):
+    p = W.base_params(**kw)
+    p.BARYON = 1
+    p.SIGMA_B = 0.8
+    p.DRAP = 0.25
+    return p
+
+
+@pytest.mark.parametrize("precision", ["float", "double"])
+def test_baryon_dtau_zero_known_answer(precision):
+    """DTAU = 0, 2-D: every sample is the cell itself, u = (1,0,0,0), so
+    n_B = J^tau = n0 * norm * 2 pi * DRAP with norm = 1/(2 pi sqrt(2 pi)
+    sigma_B e^{sigma_B^2/2}) (src/FreeStream.cpp:365-366, LandauMatch.cpp:140)
+    and V^mu = 0."""
+    p = _baryon_params(DIM_X=21, DIM_Y=23, DIM_PHIP=16, DX=0.1, DY=0.1, TAU0=0.5, DTAU=0.0)
+    rng = np.random.default_rng(4)
+    e = rng.uniform(1.0, 5.0, 21 * 23).astype(np.float32)
+    nb = rng.uniform(0.1, 0.5, 21 * 23).astype(np.float32)
+    q = O.params_finalize(p, e)
+    r = O.run(q, e, precision, nB=nb)
+    sb = 0.8
+    want = nb.astype(np.float64) * 0.25 / (np.sqrt(2 * np.pi) * sb * np.exp(sb * sb / 2))
+    sl = (slice(1, 20), slice(1, 22))
+    got = r["nB"].reshape(21, 23)
+    assert np.allclose(got[sl], want.reshape(21, 23)[sl], rtol=2e-6)
+    for c in range(4):
+        assert np.abs(r["V"][c].reshape(21, 23)[sl]).max() < 1e-6 * want.max()
+
+
+def test_baryon_current_is_the_energy_stream_of_the_same_density():
+    """2-D: the charge pass streams n_B * norm through the same spline and
+    stencils as e0/(2 pi) (src/FreestreamMilne.cpp:386), and J^mu weighs the
+    first four hypertrig rows by DRAP * d_phip instead of d_phip / TAU.  With
+    n_B chosen so that both knot densities agree, J^{tau, x, y} = TAU * DRAP *
+    T^{tau tau, tau x, tau y} (FP64 oracle)."""
+    p = _baryon_params(DIM_X=41, DIM_Y=45, DIM_PHIP=40, DX=0.1, DY=0.1, TAU0=0.1, DTAU=0.7)
+    e = W.hotspots_2d(41, 45, 0.1, 0.1, 12, 3).ravel()
+    q = O.params_finalize(p, e)
+    sb = np.float32(0.8)
+    n = np.float32(np.sqrt(2 * np.pi) * np.float64(sb) * np.exp(np.float64(sb * sb) / 2))
+    norm = 1.0 / (2 * np.pi * np.float64(n))
+    e0 = e.astype(np.float64) / np.float64(np.float32(0.197326938)) + 1e-7
+    nb = (e0 / (2 * np.pi) / norm).astype(np.float32)
+    r = O.run(q, e, "double", nB=nb, want_T=True)
+    # J from n_B and u: J^mu = V^mu + n_B u^mu
+    J = r["V"] + r["nB"][None, :] * r["u"]
+    tau = float(q.TAU)
+    scale = np.abs(r["T"][0]).max() * tau * 0.25
+    for c, tc in ((0, 0), (1, 1), (2, 2)):
+        assert np.abs(J[c] - tau * 0.25 * r["T"][tc]).max() < 1e-6 * scale
+    assert np.abs(J[3]).max() < 1e-6 * scale
