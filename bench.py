@@ -68,8 +68,13 @@ def parse():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--latency-samples", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=10)
-    ap.add_argument("--serve-seconds", type=float, default=2.0)
-    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-rates", type=str, default="200,1000,3000",
+                    help="offered request rates (per second) of the serving sweep")
+    ap.add_argument("--serve-pool", type=int, default=40, help="distinct requests in the serving pool")
+    ap.add_argument("--c3-events", type=int, default=512)
+    ap.add_argument("--c3-steps", type=int, default=6)
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
@@ -122,14 +127,15 @@ def timed(fn, steps, sync):
     return time.perf_counter() - t0
 
 
-def serve_run(fsm_amd, W, ctx, seconds, rate, seed=5):
-    """Poisson arrivals of C1-shape and mixed 2-D events through the server;
+def serve_run(fsm_amd, W, ctx, seconds, rate, pool, seed=5):
+    """Poisson arrivals at `rate` per second for `seconds` drawn from the C4
+    request pool (2-D and 3-D events, some with n_B) through the server;
     latency = submit to results in HBM."""
     rng = np.random.default_rng(seed)
-    pool = [W.config("C1")] + W.batch_mix(15, seed=seed)
     srv = fsm_amd.Server(ctx, max_batch_events=128, max_inflight=2)
-    for p, e in pool[:4]:  # build plans outside the timed run
-        rid = srv.submit(p, e)
+    # every shape once outside the timed run: plans, tile tuning, memory pools
+    for p, e, nb in pool:
+        rid = srv.submit(p, e, nb)
         while srv.poll(100000) != rid:
             pass
         srv.release(rid)
@@ -140,8 +146,8 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, seed=5):
     while True:
         now = time.perf_counter()
         if now - t0 < seconds and now >= t_next:
-            p, e = pool[rng.integers(len(pool))]
-            rid = srv.submit(p, e)
+            p, e, nb = pool[rng.integers(len(pool))]
+            rid = srv.submit(p, e, nb)
             if rid is None:
                 busy += 1
             else:
@@ -162,12 +168,52 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, seed=5):
     stats = srv.stats()
     srv.close()
     lat = np.array(lat) if lat else np.zeros(1)
-    return {"workload": "Poisson arrivals, rate %.0f/s for %.1f s: C1 events and 2-D events of mixed size "
-                        "(side 101-301, DIM_PHIP 48-200), host inputs, results kept in HBM" % (rate, seconds),
-            "completed": done, "refused_busy": busy, "events_per_s": round(done / el, 2),
+    return {"offered_per_s": rate, "completed": done, "refused_busy": busy,
+            "events_per_s": round(done / el, 2),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                            "p99": round(float(np.percentile(lat, 99)), 4)},
             "launches": stats["launches"]}
+
+
+def c3_run(fsm_amd, W, ctx, stream, steps, n_events, dev):
+    """C3: n_events independent 2-D events of mixed shapes (each its own
+    plan) as one batch, graph-replayed, on the mixed and on the FP32 path;
+    and the same events through the plain per-event FP32 path."""
+    import torch
+
+    evs = W.batch_mix(n_events, seed=7)
+    qs = [fsm_amd.params_finalize(p, e) for p, e in evs]
+    d_in = [torch.from_numpy(e).to(dev) for _, e in evs]
+    d_out = [torch.empty(17 * q.DIM, dtype=torch.float64, device=dev) for q in qs]
+    sync = torch.cuda.synchronize
+    res = {}
+    for name, path in (("mixed", fsm_amd.PATH_MIXED), ("fp32", fsm_amd.PATH_FP32)):
+        ctx.set_path(path)
+        bt = fsm_amd.Batch(ctx)
+        for q, a, o in zip(qs, d_in, d_out):
+            bt.add(q, a.data_ptr(), o.data_ptr())
+        for _ in range(2):
+            bt.run(stream)
+        res[name] = n_events * steps / timed(lambda: bt.run(stream), steps, sync)
+        bt.close()
+
+    def plain():
+        for q, a, o in zip(qs, d_in, d_out):
+            ctx.run_device(q, a.data_ptr(), o.data_ptr(), stream)
+
+    plain()
+    ps = max(1, steps // 2)
+    res["plain_fp32"] = n_events * ps / timed(plain, ps, sync)
+    cells = sum(q.DIM for q in qs)
+    samples = sum(q.DIM * q.DIM_PHIP for q in qs)
+    return {"workload": "%d 2-D events in one batch: side in {101..301}, DX 0.06-0.1, 20-300 hotspots, "
+                        "DIM_PHIP {48,100,200} (after the 2 pi DTAU/DX rule: mean %.0f), DTAU 0.4-1.2; "
+                        "device-resident, graph replay" % (n_events, samples / cells),
+            "events": n_events, "cells": cells, "samples": samples,
+            "mixed_batched_events_per_s": round(res["mixed"], 2),
+            "fp32_batched_events_per_s": round(res["fp32"], 2),
+            "plain_fp32_events_per_s": round(res["plain_fp32"], 2),
+            "mixed_batched_over_plain_fp32": round(res["mixed"] / res["plain_fp32"], 3)}
 
 
 def main():
@@ -306,10 +352,26 @@ def main():
               "stream_tflops_reference_samples": round(FLOP_PER_SAMPLE_3D * samples3 / (kt2[1] / n2 * 1e-3) / 1e12, 3)}
         del d2, o2
 
-    # ---- serving -----------------------------------------------------------------
+    # ---- C3: a batch of 512 mixed 2-D events --------------------------------------
+    c3 = None
+    if not args.no_c3:
+        ctx.set_path(fsm_amd.PATH_MIXED)
+        c3 = c3_run(fsm_amd, W, ctx, stream, args.c3_steps, args.c3_events, dev)
+
+    # ---- serving (C4 mix) ----------------------------------------------------------
     serve = None
     if not args.no_serve:
-        serve = serve_run(fsm_amd, W, ctx, args.serve_seconds, args.serve_rate)
+        ctx.set_path(fsm_amd.PATH_MIXED)
+        pool = W.serving_mix(args.serve_pool, seed=11)
+        n3 = sum(1 for p, _, _ in pool if p.DIM_ETA > 1)
+        nb = sum(1 for _, _, b in pool if b is not None)
+        sweep = [serve_run(fsm_amd, W, ctx, args.serve_seconds, float(r), pool)
+                 for r in args.serve_rates.split(",")]
+        serve = {"workload": "Poisson arrivals for %.1f s per rate from a pool of %d requests: %d 3-D (131^2 x "
+                             "DIM_ETA 41-101, DIM_RAP 21), the rest C3-style 2-D; %d carry n_B (BARYON=1, "
+                             "SIGMA_B=0.8); host inputs, results kept in HBM; mixed path"
+                             % (args.serve_seconds, len(pool), n3, nb),
+                 "sweep": sweep}
 
     # ---- CPU baseline (oracle float path = the reference algorithm) --------------
     cpu = None
@@ -357,6 +419,7 @@ def main():
                                "fp32": {k: round(v, 5) for k, v in fp32["stage_ms_per_event"].items()}},
         "plan": info,
         "c2_3d": c2,
+        "c3_batch512": c3,
         "serving": serve,
         "roofline": roofline,
         "roofline_fp32": roofline_fp32,

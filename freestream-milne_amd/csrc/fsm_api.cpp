@@ -140,7 +140,7 @@ struct Plan {
   // 3-D
   Event3D ev3{};
   DevBuf<RapSlot> slots;
-  DevBuf<float> cs;
+  DevBuf<Samp3D> samp;
   DevBuf<float> src;
   DevBuf<float> Tem;  // eta-major T
   // charge pass (BARYON = 1): normalisation of convertInitialChargeDensity;
@@ -150,7 +150,17 @@ struct Plan {
   DevBuf<RapSlot> slots_b;
   DevBuf<float> src_b;
   DevBuf<float> Jem;  // eta-major J (rows 0-3 of a 10-plane T)
+  // device bytes held by the event-sized buffers (the plan cache's budget)
+  size_t event_bytes() const {
+    return planes.n * sizeof(float4) + planes_h.n * sizeof(uint4) + src.n * sizeof(float) + Tem.n * sizeof(float) +
+           src_b.n * sizeof(float) + Jem.n * sizeof(float);
+  }
 };
+
+// Plan cache limits: a batch of many distinct event shapes (every event of a
+// mixed batch may have its own DX, DTAU and N_phi) keeps one plan each.
+const size_t kPlanCacheMax = 1024;
+const size_t kPlanCacheBytes = (size_t)8 << 30;
 
 // src/FreeStream.cpp:365-366 with the reference's float/double mixing
 // (SIGMA_B * SIGMA_B is a float product).
@@ -492,7 +502,73 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
 }
 
 // ---- 3-D plan ---------------------------------------------------------------
+// Floor of the reference's float grid index of a shifted sample
+// (src/FreeStream.cpp:163,193,198,204): x = (float)i*h + mn, x_new = x -
+// shift, floor((x_new - mn)/h); no contraction, as in the reference build.
+float ref_floor_index(int i, float shift, float mn, float h) {
+#pragma clang fp contract(off)
+  const float x = (float)i * h + mn;
+  const float xn = x - shift;
+  return floorf((xn - mn) / h);
+}
+
+// Cells i in [0, n) whose sample the reference keeps along one axis
+// (floor index in [1, n-2], src/FreeStream.cpp:213).  The floor index is
+// monotone in i, so they form one interval [*lo, *hi] (empty: *lo > *hi); g0
+// is the cell-independent floor offset, within one of the exact floor.
+void kept_interval(int n, int g0, float shift, float mn, float h, int *lo, int *hi) {
+  *lo = 1;
+  *hi = 0;
+  int i = std::max(0, 1 - g0 - 2);
+  if (i > n - 1) return;
+  while (i < n && ref_floor_index(i, shift, mn, h) < 1.0f) ++i;
+  int k = std::min(n - 1, n - 2 - g0 + 2);
+  if (k < 0) return;
+  while (k >= 0 && !(ref_floor_index(k, shift, mn, h) < (float)(n - 1))) --k;
+  if (i > k) return;
+  *lo = i;
+  *hi = k;
+}
+
+// One azimuth sample (cos c, sin s) of a slot with shift magnitude D:
+// offsets, bilinear weights and the kept intervals.  Returns -1 if a kept
+// cell would read outside [0, N] (cannot happen for grids of sane size; the
+// planner refuses the shape rather than read out of bounds).
+int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P) {
+#pragma clang fp contract(off)
+  const int nx = p.DIM_X, ny = p.DIM_Y;
+  const float xmin = (float)((-1.0) * ((double)(float)(nx - 1) / 2.0) * (double)p.DX);
+  const float ymin = (float)((-1.0) * ((double)(float)(ny - 1) / 2.0) * (double)p.DY);
+  // src/FreeStream.cpp:193-194: x_new = x - cos(phip) * D
+  const float dxj = c * D, dyj = s * D;
+  const float sx = -dxj * (1.0f / p.DX), sy = -dyj * (1.0f / p.DY);
+  const float fx = floorf(sx), fy = floorf(sy);
+  memset(P, 0, sizeof(*P));
+  P->xlo = P->ylo = 1;
+  P->xhi = P->yhi = 0;
+  // a sample shifted by more than the grid keeps nothing
+  if (!(fabsf(sx) < (float)(nx + 2)) || !(fabsf(sy) < (float)(ny + 2))) return 0;
+  P->ox = (int)fx;
+  P->oy = (int)fy;
+  const float tx = sx - fx, ty = sy - fy;
+  kept_interval(nx, P->ox, dxj, xmin, p.DX, &P->xlo, &P->xhi);
+  kept_interval(ny, P->oy, dyj, ymin, p.DY, &P->ylo, &P->yhi);
+  if (P->xlo > P->xhi || P->ylo > P->yhi) return 0;
+  if (P->xlo + P->ox < 0 || P->xhi + P->ox + 1 > nx || P->ylo + P->oy < 0 || P->yhi + P->oy + 1 > ny) return -1;
+  P->w00 = (1.0f - tx) * (1.0f - ty);
+  P->w01 = (1.0f - tx) * ty;
+  P->w10 = tx * (1.0f - ty);
+  P->w11 = tx * ty;
+  P->c = c;
+  P->s = s;
+  P->cc = c * c;
+  P->cs = c * s;
+  P->ss = s * s;
+  return 0;
+}
+
 int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
+#pragma clang fp contract(off)
   const int nx = p.DIM_X, ny = p.DIM_Y, neta = p.DIM_ETA, nrap = p.DIM_RAP, nphi = p.DIM_PHIP;
   const float DETA = p.DETA, TAU = p.TAU, TAU0 = p.TAU0, SIGMA = p.SIGMA;
   const float etamin = (float)((-1.0) * ((double)(float)(neta - 1) / 2.0) * (double)DETA);
@@ -514,6 +590,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
                    exp((-1.0) * (double)d * (double)d / (2.0 * (double)SIGMA * (double)SIGMA)));
   };
   std::vector<RapSlot> slots((size_t)neta * nrap);
+  std::vector<Samp3D> samp;
   for (int ie = 0; ie < neta; ++ie) {
     const float eta = (float)ie * DETA + etamin;
     for (int k = 0; k < nrap; ++k) {
@@ -542,21 +619,20 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
         S.th_t = (float)((1.0 / (double)TAU) * (double)th);
         S.th_tch = (float)((1.0 / (double)TAU) * (double)th / (double)ch);
         S.th2_t2 = (float)((1.0 / (double)(TAU * TAU)) * (double)th * (double)th);
-        int rmin = 1 << 30, rmax = -(1 << 30);
+        S.s0 = (int)samp.size();
         for (int j = 0; j < nphi; ++j) {
-          float sx = -(cs[2 * j] * D) * inv_dx;
-          int ox = (int)floorf(sx);
-          rmin = std::min(rmin, ox);
-          rmax = std::max(rmax, ox);
+          Samp3D P;
+          if (make_samp3d(cs[2 * j], cs[2 * j + 1], D, p, &P) < 0) return FSM_ERR_UNSUPPORTED;
+          if (P.xlo <= P.xhi && P.ylo <= P.yhi) samp.push_back(P);
         }
-        S.rmin = rmin;
-        S.rmax = rmax;
+        S.ns = (int)samp.size() - S.s0;
       }
       slots[(size_t)ie * nrap + k] = S;
     }
   }
   FSM_TRY(pl.slots.upload(slots));
-  FSM_TRY(pl.cs.upload(cs));
+  if (samp.empty()) samp.push_back(Samp3D{});  // keep a valid pointer
+  FSM_TRY(pl.samp.upload(samp));
   FSM_TRY(pl.src.ensure((size_t)p.DIM));
   FSM_TRY(pl.Tem.ensure((size_t)10 * p.DIM));
   Event3D &ev = pl.ev3;
@@ -567,7 +643,17 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.nrap = nrap;
   ev.nphi = nphi;
   ev.tiles = (nx * ny + FSM3D_TILE - 1) / FSM3D_TILE;
-  ev.lds_floats = ctx->max_lds / (int)sizeof(float);
+  {
+    // staged-row window: at least the rows one sample of a tile reads (the
+    // tile's rows + 1), else as many as fit FSM3D_WIN_FLOATS
+    const int pitch = ny + 1;
+    const int need = (FSM3D_TILE - 1) / ny + 3;
+    ev.win_rows = std::max(need, FSM3D_WIN_FLOATS / pitch);
+    ev.win_rows = std::min(ev.win_rows, nx + 2);
+    ev.win_rows = std::max(ev.win_rows, need);
+    ev.lds_floats = ev.win_rows * pitch;
+    if ((size_t)ev.lds_floats * sizeof(float) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
+  }
   ev.d_phip = (float)((2.0 * M_PI) / (double)(float)nphi);
   ev.inv_dx = inv_dx;
   ev.inv_dy = inv_dy;
@@ -577,7 +663,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.ymin = (float)((-1.0) * ((double)(float)(ny - 1) / 2.0) * (double)p.DY);
   ev.src = pl.src.p;
   ev.slots = pl.slots.p;
-  ev.cs = pl.cs.p;
+  ev.samp = pl.samp.p;
   ev.T = pl.Tem.p;
   ev.t_plane = p.DIM;
   {
@@ -644,7 +730,12 @@ int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *ou
   if (st) return st;
   pl->valid = true;
   ctx->plans.insert(ctx->plans.begin(), pl);
-  if (ctx->plans.size() > 64) ctx->plans.pop_back();
+  size_t bytes = 0;
+  for (auto &q : ctx->plans) bytes += q->event_bytes();
+  while (ctx->plans.size() > 1 && (ctx->plans.size() > kPlanCacheMax || bytes > kPlanCacheBytes)) {
+    bytes -= ctx->plans.back()->event_bytes();
+    ctx->plans.pop_back();
+  }
   *out = pl;
   return FSM_OK;
 }

@@ -158,6 +158,9 @@ struct Event2D {
 #define FSM3D_THREADS 256
 #define FSM3D_M 4
 #define FSM3D_TILE (FSM3D_THREADS * FSM3D_M)
+// LDS budget of the staged source rows of one 3-D tile (floats); the window
+// grows past it only when a tile's own rows need more.
+#define FSM3D_WIN_FLOATS 6144
 
 // Per (ieta, irap) quantities of the 3-D path.  The source rapidity plane,
 // the eta interpolation fraction and the transverse shift magnitude D depend
@@ -174,14 +177,31 @@ struct RapSlot {
   float th_t;       // tanh(rap-eta)/TAU
   float th_tch;     // tanh(rap-eta)/(TAU cosh(rap-eta))
   float th2_t2;     // tanh^2(rap-eta)/TAU^2
-  int rmin, rmax;   // row offset range touched by this slot (min/max floor offset in x)
+  int s0, ns;       // this slot's live azimuth samples: Samp3D [s0, s0+ns)
   int pad0, pad1;
 };
+
+// One azimuth sample of a 3-D rapidity slot that keeps at least one cell.
+// The shift (cos phi_j, sin phi_j) * D is the same for every cell, so the
+// floor offset and the bilinear weights are per-sample constants.  The
+// reference keeps a sample iff the floor of its float grid index lies in
+// [1, N-2] (src/FreeStream.cpp:213); that floor is monotone in the cell
+// index, so the kept cells form one interval per axis, found on the host with
+// the reference's own float expressions.
+struct Samp3D {
+  int ox, oy;                // floor offset (cells) of the shifted point
+  int xlo, xhi, ylo, yhi;    // kept cells: ix in [xlo, xhi], iy in [ylo, yhi]
+  float w00, w01, w10, w11;  // bilinear weights of Q[ix+ox][iy+oy], [..][iy+oy+1], [ix+ox+1][..], [..+1][..+1]
+  float c, s, cc, cs, ss;    // cos phi_j, sin phi_j, c*c, c*s, s*s
+  float pad;
+};
+static_assert(sizeof(Samp3D) == 64, "Samp3D is read with one scalar load");
 
 struct Event3D {
   int nx, ny, neta, nrap, nphi;
   int tiles;            // transverse tiles per eta plane
-  int lds_floats;       // LDS floats available for the staged plane
+  int lds_floats;       // LDS floats of the staged-row window (win_rows * (NY+1))
+  int win_rows;         // rows of the window
   float d_phip;
   float inv_dx, inv_dy; // 1/DX, 1/DY: shift in cells = -(cos(phi) * D) * inv_dx
   float dx, dy;
@@ -189,7 +209,7 @@ struct Event3D {
   const float *e_gev;   // input GeV/fm^3, reference layout
   float *src;           // eta-major e0*norm planes
   const RapSlot *slots; // [neta][nrap]
-  const float *cs;      // [2*nphi]: cos(phi_j), sin(phi_j)
+  const Samp3D *samp;   // live samples of every slot
   float *T;             // 10 planes, eta-major layout
   long t_plane;
   float norm_factor;    // energy: convertInitialDensity norm; charge pass: convertInitialChargeDensity norm

@@ -316,33 +316,37 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ in,
   }
 }
 
-// Q(x, y) = (1-f) rf0 E[eta0] + f rf1 E[eta0+1], zero outside the grid.
-__device__ __forceinline__ float q_global(const float *p0, const float *p1, float w0, float w1, float r0f,
-                                          float r1f, int x, int y, int nx, int ny) {
-  if (x < 0 || x >= nx || y < 0 || y >= ny) return 0.0f;
-  const long a = (long)x * ny + y;
-  return w0 * (p0[a] * r0f) + w1 * (p1[a] * r1f);
-}
-
-// One transverse tile (FSM3D_TILE consecutive cells of the ix*NY+iy order)
-// of one eta plane.  For each valid rapidity slot the eta-blended source plane
-// rows the tile can reach are staged in LDS with row pitch NY+1 and one zero
-// row/column of halo, so a sample that the boundary rule keeps with a
-// neighbouring floor still reads in-bounds (its value is continuous there).
+// One transverse tile (FSM3D_TILE consecutive cells of the ix*NY+iy order,
+// i.e. a few whole rows) of one eta plane.  Per rapidity slot the kernel walks
+// the slot's live azimuth samples (host tables: offset, bilinear weights and
+// the interval of cells the reference keeps along each axis).  The source the
+// samples read is the eta-blended plane
+//   Q(x, y) = (1-f) rf0 E[eta0](x, y) + f rf1 E[eta0+1](x, y),
+// staged in LDS as a window of whole rows (pitch NY+1: column NY and any row
+// outside the grid are zero).  A window covers the rows a run of consecutive
+// samples needs for this tile and is restaged when the next sample falls
+// outside it, so a slot whose samples only graze the grid stages only what
+// they read.  Every read lies in the window: a kept cell's source rows
+// ix+ox, ix+ox+1 lie in [0, NX] and columns iy+oy, iy+oy+1 in [0, NY]
+// (checked on the host).
 __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int nx = ev.nx, ny = ev.ny, nxy = nx * ny;
   const int pitch = ny + 1;
+  const int W = ev.win_rows;
   const int c0 = tile * FSM3D_TILE;
   const int row_lo = c0 / ny;
   const int row_hi = min(c0 + FSM3D_TILE - 1, nxy - 1) / ny;
-  int cx[FSM3D_M], cy[FSM3D_M];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int cx[FSM3D_M], cy[FSM3D_M], ba[FSM3D_M];
   bool live[FSM3D_M];
 #pragma unroll
   for (int m = 0; m < FSM3D_M; ++m) {
     int c = c0 + threadIdx.x + FSM3D_THREADS * m;
     live[m] = c < nxy;
+    if (!live[m]) c = c0;
     cx[m] = c / ny;
     cy[m] = c - cx[m] * ny;
+    ba[m] = cx[m] * pitch + cy[m];
   }
   float T[FSM3D_M][10];
 #pragma unroll
@@ -352,72 +356,69 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
 
   for (int k = 0; k < ev.nrap; ++k) {
     const RapSlot S = ev.slots[(long)ieta * ev.nrap + k];
-    if (!S.valid) continue;
-    // rows [r0, r1] (r0 may be -1, r1 may be nx: zero rows)
-    const int r0 = max(-1, row_lo + S.rmin - 1);
-    const int r1 = min(nx, row_hi + S.rmax + 2);
-    if (r0 > r1) continue;
-    const int nrows = r1 - r0 + 1;
-    const bool use_lds = (long)nrows * pitch <= (long)ev.lds_floats;
+    if (S.ns == 0) continue;
     const float w0 = 1.0f - S.feta, w1 = S.feta;
     const float *p0 = ev.src + (long)S.eta0 * nxy;
     const float *p1 = ev.src + (long)(S.eta0 + 1) * nxy;
-    __syncthreads();
-    if (use_lds) {
-      for (int e = threadIdx.x; e < nrows * pitch; e += blockDim.x) {
-        const int r = e / pitch, y = e - r * pitch;
-        q[e] = q_global(p0, p1, w0, w1, S.rf0, S.rf1, r0 + r, y, nx, ny);
-      }
-    }
-    __syncthreads();
     float mo[FSM3D_M][6];
 #pragma unroll
     for (int m = 0; m < FSM3D_M; ++m)
 #pragma unroll
       for (int c = 0; c < 6; ++c) mo[m][c] = 0.0f;
-    for (int j = 0; j < ev.nphi; ++j) {
-      const float c = ev.cs[2 * j], s = ev.cs[2 * j + 1];
-      // src/FreeStream.cpp:193-194: x_new = x - cos(phip) * D
-      const float dxj = c * S.D, dyj = s * S.D;
-      const float sx = -dxj * ev.inv_dx;
-      const float sy = -dyj * ev.inv_dy;
-      const float fx = floorf(sx), fy = floorf(sy);
-      const int ox = (int)fx, oy = (int)fy;
-      const float tx = sx - fx, ty = sy - fy;
-      // a sample is kept only if its floor index is in [1, nx-2], which
-      // needs ix+ox in [0, nx-1]
-      if (row_hi + ox < 0 || row_lo + ox > nx - 1) continue;
-      const float cc = c * c, csv = c * s, ss = s * s;
+    int wlo = 1, whi = 0;  // empty window
+    const Samp3D *sp = ev.samp + S.s0;
+    for (int j = 0; j < S.ns; ++j) {
+      const Samp3D P = sp[j];
+      const int a = max(row_lo, P.xlo), b = min(row_hi, P.xhi);
+      if (a > b) continue;  // no kept cell in this tile's rows
+      if (a + P.ox < wlo || b + P.ox + 1 > whi) {
+        // new window: the rows of this and the following samples while
+        // they fit W rows (uniform over the workgroup)
+        int lo = a + P.ox, hi = b + P.ox + 1;
+        for (int j2 = j + 1; j2 < S.ns; ++j2) {
+          const Samp3D P2 = sp[j2];
+          const int a2 = max(row_lo, P2.xlo), b2 = min(row_hi, P2.xhi);
+          if (a2 > b2) continue;
+          const int l2 = min(lo, a2 + P2.ox), h2 = max(hi, b2 + P2.ox + 1);
+          if (h2 - l2 + 1 > W) break;
+          lo = l2;
+          hi = h2;
+        }
+        __syncthreads();  // previous window no longer read
+        wlo = lo;
+        whi = hi;
+        for (int r = wave; r <= hi - lo; r += FSM3D_THREADS / 64) {
+          const int gx = lo + r;
+          const bool rin = gx >= 0 && gx < nx;
+          const float *r0p = p0 + (long)gx * ny, *r1p = p1 + (long)gx * ny;
+          float *qr = q + r * pitch;
+          for (int y = lane; y < pitch; y += 64) {
+            float v = 0.0f;
+            if (rin && y < ny) v = w0 * (r0p[y] * S.rf0) + w1 * (r1p[y] * S.rf1);
+            qr[y] = v;
+          }
+        }
+        __syncthreads();
+      }
+      const int delta = (P.ox - wlo) * pitch + P.oy;
+      const bool xall = a == row_lo && b == row_hi;  // every row of the tile is kept along x
+      const unsigned xs = (unsigned)(P.xhi - P.xlo), ys = (unsigned)(P.yhi - P.ylo);
 #pragma unroll
       for (int m = 0; m < FSM3D_M; ++m) {
-        const int sxi = cx[m] + ox, syi = cy[m] + oy;
-        const bool ok = live[m] && fsm_mask_ok(cx[m], sxi, dxj, ev.xmin, ev.dx, nx) &&
-                        fsm_mask_ok(cy[m], syi, dyj, ev.ymin, ev.dy, ny);
-        float v = 0.0f;
+        const bool ok = live[m] && (unsigned)(cy[m] - P.ylo) <= ys && (xall || (unsigned)(cx[m] - P.xlo) <= xs);
         if (ok) {
-          float q00, q01, q10, q11;
-          if (use_lds) {
-            const int a = (sxi - r0) * pitch + syi;
-            q00 = q[a];
-            q01 = q[a + 1];
-            q10 = q[a + pitch];
-            q11 = q[a + pitch + 1];
-          } else {
-            q00 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi, syi, nx, ny);
-            q01 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi, syi + 1, nx, ny);
-            q10 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi + 1, syi, nx, ny);
-            q11 = q_global(p0, p1, w0, w1, S.rf0, S.rf1, sxi + 1, syi + 1, nx, ny);
-          }
-          const float l0 = fmaf(ty, q01 - q00, q00);
-          const float l1 = fmaf(ty, q11 - q10, q10);
-          v = fmaf(tx, l1 - l0, l0);
+          const float *qa = q + ba[m] + delta;
+          float v = qa[0] * P.w00;
+          v = fmaf(qa[1], P.w01, v);
+          v = fmaf(qa[pitch], P.w10, v);
+          v = fmaf(qa[pitch + 1], P.w11, v);
+          mo[m][0] += v;
+          mo[m][1] = fmaf(v, P.c, mo[m][1]);
+          mo[m][2] = fmaf(v, P.s, mo[m][2]);
+          mo[m][3] = fmaf(v, P.cc, mo[m][3]);
+          mo[m][4] = fmaf(v, P.cs, mo[m][4]);
+          mo[m][5] = fmaf(v, P.ss, mo[m][5]);
         }
-        mo[m][0] += v;
-        mo[m][1] = fmaf(v, c, mo[m][1]);
-        mo[m][2] = fmaf(v, s, mo[m][2]);
-        mo[m][3] = fmaf(v, cc, mo[m][3]);
-        mo[m][4] = fmaf(v, csv, mo[m][4]);
-        mo[m][5] = fmaf(v, ss, mo[m][5]);
       }
     }
     // hypertrig rows (src/LandauMatch.cpp:46-55) times the Jacobian (:93,101)

@@ -16,6 +16,10 @@ Random draws use numpy's PCG64 with the seed recorded in each config.
       hotspots times a flat-top envelope in eta (flat for |eta| < 2, Gaussian
       tails of width 0.5); SIGMA = 1, DIM_RAP = 21, N_phi = 48, DTAU = 0.8
   C3  batch of 2-D events of mixed sizes (see ``batch_mix``)
+  C4  serving mix: 80% C3-style 2-D events, 20% C2-style 3-D events with
+      DIM_ETA uniform in 41..101; 25% of requests also carry n_B
+      (BARYON = 1, SIGMA_B = 0.8, n_B = 0.1 x normalised energy profile)
+      (see ``serving_mix``)
 """
 from __future__ import annotations
 
@@ -107,6 +111,15 @@ def config(name: str, seed: int | None = None):
     raise KeyError(name)
 
 
+def c2_like(neta: int = 101, seed: int = 2):
+    """A C2-style 3-D event with DIM_ETA = neta (the 3-D requests of C4)."""
+    p = base_params(DIM_X=131, DIM_Y=131, DIM_ETA=neta, DIM_RAP=21, DIM_PHIP=48, DX=0.1, DY=0.1,
+                    DETA=0.1, TAU0=0.1, DTAU=0.8, SIGMA=1.0)
+    e2 = hotspots_2d(131, 131, 0.1, 0.1, 100, seed)
+    env = eta_envelope(neta, 0.1)
+    return p, (e2[:, :, None] * env[None, None, :]).astype(np.float32).ravel()
+
+
 def small_3d(nx: int = 41, neta: int = 21, nrap: int = 9, nphi: int = 24, dtau: float = 0.5, seed: int = 5):
     """A reduced 3-D event for parity tests (seconds on the CPU oracle)."""
     p = base_params(DIM_X=nx, DIM_Y=nx, DIM_ETA=neta, DIM_RAP=nrap, DIM_PHIP=nphi, DX=0.1, DY=0.1,
@@ -146,3 +159,23 @@ def with_baryon(p: Params, sigma_b: float = 0.8) -> Params:
     q.BARYON = 1
     q.SIGMA_B = sigma_b
     return q
+
+
+def serving_mix(n_events: int, seed: int = 11):
+    """C4 request pool: list of (params, e, nB or None).  80% C3-style 2-D
+    events, 20% C2-style 3-D events with DIM_ETA uniform in 41..101; 25% of
+    requests carry a baryon density (BARYON = 1, SIGMA_B = 0.8)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    twod = batch_mix(n_events, seed=seed)
+    out = []
+    for k in range(n_events):
+        if rng.uniform() < 0.2:
+            p, e = c2_like(int(rng.integers(41, 102)), seed=seed * 1009 + k)
+        else:
+            p, e = twod[k]
+        nb = None
+        if rng.uniform() < 0.25:
+            p = with_baryon(p)
+            nb = baryon_density(e)
+        out.append((p, e, nb))
+    return out
