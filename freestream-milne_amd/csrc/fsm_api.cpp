@@ -141,6 +141,9 @@ struct Plan {
   Event3D ev3{};
   DevBuf<RapSlot> slots;
   DevBuf<Samp3D> samp;
+  DevBuf<int> eta_order;
+  DevBuf<int2> srange;
+  std::vector<float> eta_work;  // kept cell-samples of each eta plane (work estimate)
   DevBuf<float> src;
   DevBuf<float> Tem;  // eta-major T
   // charge pass (BARYON = 1): normalisation of convertInitialChargeDensity;
@@ -626,11 +629,54 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
           if (P.xlo <= P.xhi && P.ylo <= P.yhi) samp.push_back(P);
         }
         S.ns = (int)samp.size() - S.s0;
+        // by source row offset: a tile's samples form one index range and
+        // consecutive samples read neighbouring rows
+        std::stable_sort(samp.begin() + S.s0, samp.end(), [](const Samp3D &u, const Samp3D &v) {
+          return u.ox < v.ox || (u.ox == v.ox && u.oy < v.oy);
+        });
       }
       slots[(size_t)ie * nrap + k] = S;
     }
   }
   FSM_TRY(pl.slots.upload(slots));
+  // work per eta plane: kept cell-samples of its slots; the one-event launch
+  // starts the heaviest planes first
+  pl.eta_work.assign(neta, 0.0f);
+  for (int ie = 0; ie < neta; ++ie)
+    for (int k = 0; k < nrap; ++k) {
+      const RapSlot &S = slots[(size_t)ie * nrap + k];
+      for (int j = S.s0; j < S.s0 + S.ns; ++j)
+        pl.eta_work[ie] += (float)(samp[j].xhi - samp[j].xlo + 1) * (float)(samp[j].yhi - samp[j].ylo + 1);
+    }
+  {
+    std::vector<int> order(neta);
+    for (int ie = 0; ie < neta; ++ie) order[ie] = ie;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pl.eta_work[x] > pl.eta_work[y]; });
+    FSM_TRY(pl.eta_order.upload(order));
+  }
+  {
+    // per (slot, tile): the index range of the slot's samples that keep a
+    // cell in the tile's rows (tiles are FSM3D_TILE consecutive cells)
+    const int tiles = (nx * ny + FSM3D_TILE - 1) / FSM3D_TILE;
+    std::vector<int2> rg((size_t)neta * nrap * tiles, make_int2(0, 0));
+    for (size_t si = 0; si < slots.size(); ++si) {
+      const RapSlot &S = slots[si];
+      for (int t = 0; t < tiles; ++t) {
+        const int c0 = t * FSM3D_TILE;
+        const int rlo = c0 / ny, rhi = std::min(c0 + FSM3D_TILE - 1, nx * ny - 1) / ny;
+        int first = -1, last = -1;
+        for (int j = 0; j < S.ns; ++j) {
+          const Samp3D &P = samp[S.s0 + j];
+          if (std::max(rlo, P.xlo) <= std::min(rhi, P.xhi)) {
+            if (first < 0) first = j;
+            last = j;
+          }
+        }
+        if (first >= 0) rg[si * tiles + t] = make_int2(first, last + 1);
+      }
+    }
+    FSM_TRY(pl.srange.upload(rg));
+  }
   if (samp.empty()) samp.push_back(Samp3D{});  // keep a valid pointer
   FSM_TRY(pl.samp.upload(samp));
   FSM_TRY(pl.src.ensure((size_t)p.DIM));
@@ -664,6 +710,8 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.src = pl.src.p;
   ev.slots = pl.slots.p;
   ev.samp = pl.samp.p;
+  ev.srange = pl.srange.p;
+  ev.eta_order = pl.eta_order.p;
   ev.T = pl.Tem.p;
   ev.t_plane = p.DIM;
   {
@@ -1382,6 +1430,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   std::vector<Event3D> e3;
   std::vector<EventLandau> el;
   std::vector<int4> ks, sp, s2, src, s3, ld;
+  std::vector<float> s3w;  // work estimate of each 3-D stream item
   size_t bytes = 0;
   for (auto &ev : b->evs)
     if (!ev.ws) bytes += event_workspace_bytes_all(ev.p, ev.d_nb != nullptr);
@@ -1426,7 +1475,10 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     for (int x = 0; x < (nxy + 63) / 64; ++x)
       for (int y = 0; y < (p.DIM_ETA + 63) / 64; ++y) src.push_back(make_int4(i, x, y, 0));
     for (int ie = 0; ie < p.DIM_ETA; ++ie)
-      for (int t = 0; t < d.tiles; ++t) s3.push_back(make_int4(i, t, ie, 0));
+      for (int t = 0; t < d.tiles; ++t) {
+        s3.push_back(make_int4(i, t, ie, 0));
+        s3w.push_back(ev.plan->eta_work[ie]);
+      }
     b->lds3 = std::max(b->lds3, (size_t)d.lds_floats * sizeof(float));
     *eta_major = 1;
     return (const float *)d.T;
@@ -1449,6 +1501,16 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     }
     el.push_back(L);
     for (int c = 0; c < p.DIM; c += kLandauChunk) ld.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
+  }
+  {
+    // 3-D stream items by decreasing work (every event's heavy eta planes
+    // first), so the tail of the launch is made of short items
+    std::vector<int> idx(s3.size());
+    for (size_t k = 0; k < idx.size(); ++k) idx[k] = (int)k;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return s3w[x] > s3w[y]; });
+    std::vector<int4> sorted(s3.size());
+    for (size_t k = 0; k < idx.size(); ++k) sorted[k] = s3[idx[k]];
+    s3.swap(sorted);
   }
   std::vector<int4> items;
   auto put = [&](std::vector<int4> &v, int *off, int *n) {

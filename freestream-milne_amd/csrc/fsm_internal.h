@@ -209,7 +209,10 @@ struct Event3D {
   const float *e_gev;   // input GeV/fm^3, reference layout
   float *src;           // eta-major e0*norm planes
   const RapSlot *slots; // [neta][nrap]
-  const Samp3D *samp;   // live samples of every slot
+  const Samp3D *samp;   // live samples of every slot, each slot's sorted by (ox, oy)
+  const int2 *srange;   // [neta*nrap][tiles]: samples [x, y) of the slot (relative to s0) that keep a
+                        // cell in the tile's rows
+  const int *eta_order; // eta planes by decreasing work (the one-event launch's order)
   float *T;             // 10 planes, eta-major layout
   long t_plane;
   float norm_factor;    // energy: convertInitialDensity norm; charge pass: convertInitialChargeDensity norm

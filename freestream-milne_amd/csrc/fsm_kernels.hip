@@ -367,7 +367,8 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
       for (int c = 0; c < 6; ++c) mo[m][c] = 0.0f;
     int wlo = 1, whi = 0;  // empty window
     const Samp3D *sp = ev.samp + S.s0;
-    for (int j = 0; j < S.ns; ++j) {
+    const int2 rg = ev.srange[((long)ieta * ev.nrap + k) * ev.tiles + tile];
+    for (int j = rg.x; j < rg.y; ++j) {
       const Samp3D P = sp[j];
       const int a = max(row_lo, P.xlo), b = min(row_hi, P.xhi);
       if (a > b) continue;  // no kept cell in this tile's rows
@@ -375,11 +376,12 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
         // new window: the rows of this and the following samples while
         // they fit W rows (uniform over the workgroup)
         int lo = a + P.ox, hi = b + P.ox + 1;
-        for (int j2 = j + 1; j2 < S.ns; ++j2) {
-          const Samp3D P2 = sp[j2];
-          const int a2 = max(row_lo, P2.xlo), b2 = min(row_hi, P2.xhi);
+        for (int j2 = j + 1; j2 < rg.y; ++j2) {
+          const int2 xr = *reinterpret_cast<const int2 *>(&sp[j2].xlo);
+          const int ox2 = sp[j2].ox;
+          const int a2 = max(row_lo, xr.x), b2 = min(row_hi, xr.y);
           if (a2 > b2) continue;
-          const int l2 = min(lo, a2 + P2.ox), h2 = max(hi, b2 + P2.ox + 1);
+          const int l2 = min(lo, a2 + ox2), h2 = max(hi, b2 + ox2 + 1);
           if (h2 - l2 + 1 > W) break;
           lo = l2;
           hi = h2;
@@ -400,25 +402,34 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
         }
         __syncthreads();
       }
+      // branch-free over the lane's cells: a dropped cell reads element 0 of
+      // the window and its value is discarded
       const int delta = (P.ox - wlo) * pitch + P.oy;
-      const bool xall = a == row_lo && b == row_hi;  // every row of the tile is kept along x
       const unsigned xs = (unsigned)(P.xhi - P.xlo), ys = (unsigned)(P.yhi - P.ylo);
+      bool ok[FSM3D_M];
+      float q00[FSM3D_M], q01[FSM3D_M], q10[FSM3D_M], q11[FSM3D_M];
 #pragma unroll
       for (int m = 0; m < FSM3D_M; ++m) {
-        const bool ok = live[m] && (unsigned)(cy[m] - P.ylo) <= ys && (xall || (unsigned)(cx[m] - P.xlo) <= xs);
-        if (ok) {
-          const float *qa = q + ba[m] + delta;
-          float v = qa[0] * P.w00;
-          v = fmaf(qa[1], P.w01, v);
-          v = fmaf(qa[pitch], P.w10, v);
-          v = fmaf(qa[pitch + 1], P.w11, v);
-          mo[m][0] += v;
-          mo[m][1] = fmaf(v, P.c, mo[m][1]);
-          mo[m][2] = fmaf(v, P.s, mo[m][2]);
-          mo[m][3] = fmaf(v, P.cc, mo[m][3]);
-          mo[m][4] = fmaf(v, P.cs, mo[m][4]);
-          mo[m][5] = fmaf(v, P.ss, mo[m][5]);
-        }
+        ok[m] = live[m] & ((unsigned)(cy[m] - P.ylo) <= ys) & ((unsigned)(cx[m] - P.xlo) <= xs);
+        const int ad = ok[m] ? ba[m] + delta : 0;
+        q00[m] = q[ad];
+        q01[m] = q[ad + 1];
+        q10[m] = q[ad + pitch];
+        q11[m] = q[ad + pitch + 1];
+      }
+#pragma unroll
+      for (int m = 0; m < FSM3D_M; ++m) {
+        float v = q00[m] * P.w00;
+        v = fmaf(q01[m], P.w01, v);
+        v = fmaf(q10[m], P.w10, v);
+        v = fmaf(q11[m], P.w11, v);
+        v = ok[m] ? v : 0.0f;
+        mo[m][0] += v;
+        mo[m][1] = fmaf(v, P.c, mo[m][1]);
+        mo[m][2] = fmaf(v, P.s, mo[m][2]);
+        mo[m][3] = fmaf(v, P.cc, mo[m][3]);
+        mo[m][4] = fmaf(v, P.cs, mo[m][4]);
+        mo[m][5] = fmaf(v, P.ss, mo[m][5]);
       }
     }
     // hypertrig rows (src/LandauMatch.cpp:46-55) times the Jacobian (:93,101)
@@ -449,9 +460,9 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
 
 __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_one(Event3D ev) {
   extern __shared__ float qlds[];
-  // round-robin over the XCDs: the work of an eta plane depends on how many
-  // of its rapidity slots are valid, so contiguous ranges would unbalance
-  stream3d_tile(ev, blockIdx.x % ev.tiles, blockIdx.x / ev.tiles, qlds);
+  // eta planes in decreasing order of work, so the last workgroups to start
+  // are the short ones; tiles of one plane round-robin over the XCDs
+  stream3d_tile(ev, blockIdx.x % ev.tiles, ev.eta_order[blockIdx.x / ev.tiles], qlds);
 }
 
 __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D *__restrict__ evs,
@@ -459,7 +470,7 @@ __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D 
                                                                   int n_items) {
   extern __shared__ float qlds[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];  // round-robin over the XCDs (see k_stream3d_one)
+    int4 item = items[it];  // the host orders items by decreasing work
     const Event3D ev = evs[item.x];
     stream3d_tile(ev, item.y, item.z, qlds);
   }
