@@ -1402,7 +1402,7 @@ struct fsm_batch {
   // work-item ranges per stage: knot scale, spline, 2-D stream, 3-D source, 3-D stream, Landau
   int off_ks = 0, n_ks = 0, off_sp = 0, n_sp = 0, off_s2 = 0, n_s2 = 0;
   int off_src = 0, n_src = 0, off_s3 = 0, n_s3 = 0, off_l = 0, n_l = 0;
-  size_t lds2 = 0, lds3 = 0;
+  size_t lds2 = 0, lds3 = 0, lds_l = 0;  // dynamic LDS of the stream and Landau launches
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // stream-ordered uploads (used by the server): descriptors go through
@@ -1436,7 +1436,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     if (!ev.ws) bytes += event_workspace_bytes_all(ev.p, ev.d_nb != nullptr);
   if (bytes) FSM_TRY(b->arena.ensure(bytes));
   char *arena_cur = b->arena.p;
-  b->lds2 = b->lds3 = 0;
+  b->lds2 = b->lds3 = b->lds_l = 0;
   const int path = b->ctx->path;
   // stage-A work items of one streaming pass (energy, or the charge pass of a
   // baryon event) in the workspace at `cur`; returns its T (or J) and layout
@@ -1500,7 +1500,16 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       L.out_b = ev.d_out + (size_t)FSM_NUM_OUTPUTS * p.DIM;
     }
     el.push_back(L);
-    for (int c = 0; c < p.DIM; c += kLandauChunk) ld.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
+    if (em && p.DIM_ETA > 1) {
+      // 3-D: eta x transverse tiles of the LDS-transposed Landau form
+      const int nxy = p.DIM_X * p.DIM_Y;
+      for (int et = 0; et < (p.DIM_ETA + FSML_TE - 1) / FSML_TE; ++et)
+        for (int xt = 0; xt < (nxy + FSML_TX - 1) / FSML_TX; ++xt) ld.push_back(make_int4(li, xt, et, 1));
+      b->lds_l = std::max(b->lds_l, fsm_landau_lds_bytes(ev.d_nb != nullptr));
+    } else {
+      for (int c = 0; c < p.DIM; c += kLandauChunk)
+        ld.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
+    }
   }
   {
     // 3-D stream items by decreasing work (every event's heavy eta planes
@@ -1564,7 +1573,7 @@ int batch_launch(fsm_batch *b, hipStream_t s, bool timed) {
   FSM_TRY(fsm_launch_stream2d_batch(b->d_ev2.p, it + b->off_s2, b->n_s2, b->lds2, mfma, s));
   FSM_TRY(fsm_launch_stream3d_batch(b->d_ev3.p, it + b->off_s3, b->n_s3, b->lds3, s));
   if (timed) tmark(ctx, 2, s);
-  FSM_TRY(fsm_launch_landau_batch(b->d_evl.p, it + b->off_l, b->n_l, s));
+  FSM_TRY(fsm_launch_landau_batch(b->d_evl.p, it + b->off_l, b->n_l, b->lds_l, s));
   if (timed) tmark(ctx, 3, s);
   return FSM_OK;
 }

@@ -220,6 +220,13 @@ struct Event3D {
 };
 
 // --------------------------------------------------------------- Landau ----
+#define FSM_NUM_OUTPUTS 17
+// 3-D tile of the Landau kernel: FSML_TE eta planes x FSML_TX transverse
+// cells (one cell per thread); outputs staged in LDS with a padded pitch.
+#define FSML_TE 16
+#define FSML_TX 16
+#define FSML_PITCH (FSML_TE + 1)
+#define FSML_PLANE (FSML_TX * FSML_PITCH)
 struct EventLandau {
   int dim;
   int nxy, neta;        // for the eta-major T layout

@@ -36,5 +36,8 @@ hipError_t fsm_launch_stream2d_batch(const Event2D *evs, const int4 *items, int 
                                      hipStream_t s);
 hipError_t fsm_launch_src3d_batch(const Event3D *evs, const int4 *items, int n, hipStream_t s);
 hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s);
-hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s);
+// Dynamic LDS of the 3-D Landau tiles (17 output planes, 22 with baryon).
+size_t fsm_landau_lds_bytes(int baryon);
+hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, size_t lds_bytes,
+                                   hipStream_t s);
 #endif  // FSM_KERNELS_H_

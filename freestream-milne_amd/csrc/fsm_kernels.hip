@@ -495,6 +495,18 @@ __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D 
 // reference's fallback applies (e = 1e-3, u = (1,0,0,0) written).
 // tol2: squared relative off-diagonal residual of convergence; loose2: the
 // residual still accepted after the last sweep.
+// Reciprocal square root for the Jacobi rotations: the hardware estimate in
+// FP32 (1 ulp), refined by two Newton steps in FP64 (each doubles the bits).
+// Arguments are positive and finite.
+__device__ __forceinline__ float jrsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double jrsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double r = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, r, y);
+  r = fma(-(x * y), y, 1.0);
+  return fma(0.5 * y, r, y);
+}
+
 template <typename R>
 __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4]) {
   const R tau = (R)TAU;
@@ -524,30 +536,31 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
     for (int i = 1; i < 4; ++i) {
       const R b = A[0][i];
       if (b == (R)0) continue;
-      const R s = A[0][0] + A[i][i];
+      const R s = A[0][0] + A[i][i], d = A[0][0] - A[i][i];
       const R disc = (s - (R)2 * fabs(b)) * (s + (R)2 * fabs(b));
       if (!(s > (R)0) || !(disc > (R)0)) {
         ok = false;
         break;
       }
-      const R rq = (R)1 / sqrt(disc);
+      const R rq = jrsq(disc);
+      const R sq = disc * rq;        // sqrt(disc)
       const R ch2 = s * rq;          // cosh 2theta
       const R sh2 = (R)-2 * b * rq;  // sinh 2theta
-      const R ch = sqrt((R)0.5 * (ch2 + (R)1));
-      const R sh = sh2 / ((R)2 * ch);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const R a0 = A[0][k], ai = A[i][k];
-        A[0][k] = ch * a0 + sh * ai;
-        A[i][k] = sh * a0 + ch * ai;
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const R a0 = A[k][0], ai = A[k][i];
-        A[k][0] = ch * a0 + sh * ai;
-        A[k][i] = sh * a0 + ch * ai;
-      }
+      const R x = (R)0.5 * (ch2 + (R)1);
+      const R rx = jrsq(x);
+      const R ch = x * rx;                // cosh theta = sqrt(x)
+      const R sh = (R)0.5 * sh2 * rx;     // sinh theta = sinh 2theta / (2 cosh theta)
+      // the boosted 2x2 block in closed form: diag((d + sq)/2, (sq - d)/2)
+      A[0][0] = (R)0.5 * (d + sq);
+      A[i][i] = (R)0.5 * (sq - d);
       A[0][i] = A[i][0] = (R)0;
+#pragma unroll
+      for (int k = 1; k < 4; ++k) {
+        if (k == i) continue;
+        const R a0 = A[0][k], ai = A[i][k];
+        A[0][k] = A[k][0] = ch * a0 + sh * ai;
+        A[i][k] = A[k][i] = sh * a0 + ch * ai;
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const R m0 = M[k][0], mi = M[k][i];
@@ -580,28 +593,21 @@ __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_
   (void)landau_jacobi<double>(t, TAU, 1.0e-28, 1.0e-12, e_out, u_out);
 }
 
-// Mixed precision: FP32 sweeps, and FP64 only where FP32 cannot decide the
-// result to the parity tolerance -- a failed or unconverged FP32 solve, fast
-// flow (u^tau > 1.5, the frame is conditioned like gamma^2) and cells whose
-// e is a small difference of large terms (kappa > 16, nearly null tensors in
-// dilute edge cells), where FP32 rounding in T is amplified by kappa.
-__device__ void landau_mixed(const float t[10], float TAU, float *e_out, float u_out[4]) {
-  float e32, u32[4];
-  const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, &e32, u32);
+// FP32 stage of the mixed path.  Returns true when FP32 decides the result to
+// the parity tolerance; false for a failed or unconverged solve, fast flow
+// (u^tau > 1.5: the frame is conditioned like gamma^2) and cells whose e is a
+// small difference of large terms (kappa > 16: nearly null tensors in dilute
+// edge cells, where FP32 rounding in T is amplified by kappa).  Those cells
+// are solved again in FP64.
+__device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
   // First-order sensitivity of e = T_{mu nu} u^mu u^nu to a relative change
   // of every component: kappa = sum |u_mu| |T^{mu nu}| |u_nu| / e (>= u0^2 T^tt / e).
-  const float a0 = fabsf(u32[0]), a1 = fabsf(u32[1]), a2 = fabsf(u32[2]), a3 = TAU * TAU * fabsf(u32[3]);
+  const float a0 = fabsf(u_out[0]), a1 = fabsf(u_out[1]), a2 = fabsf(u_out[2]), a3 = TAU * TAU * fabsf(u_out[3]);
   const float S = a0 * a0 * fabsf(t[0]) + a1 * a1 * fabsf(t[4]) + a2 * a2 * fabsf(t[7]) + a3 * a3 * fabsf(t[9]) +
                   2.0f * (a0 * (a1 * fabsf(t[1]) + a2 * fabsf(t[2]) + a3 * fabsf(t[3])) +
                           a1 * (a2 * fabsf(t[5]) + a3 * fabsf(t[6])) + a2 * a3 * fabsf(t[8]));
-  const bool need64 = st != 1 || !(u32[0] <= 1.5f) || !(S <= 16.0f * e32);
-  if (need64) {
-    landau_fp64(t, TAU, e_out, u_out);
-  } else {
-    *e_out = e32;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) u_out[c] = u32[c];
-  }
+  return st == 1 && u_out[0] <= 1.5f && S <= 16.0f * *e_out;
 }
 
 __device__ float eos_pressure(float e, int eos) {
@@ -642,100 +648,235 @@ __device__ float eos_pressure(float e, int eos) {
   return 0.0f;
 }
 
-__device__ void landau_cell_out(const EventLandau &ev, long is) {
-#pragma clang fp contract(off)
-  long tidx = is;
-  if (ev.t_eta_major) {
-    const long xy = is / ev.neta;
-    const long ie = is - xy * ev.neta;
-    tidx = ie * ev.nxy + xy;
+// Results of one cell in the reference's float precision.
+struct CellOut {
+  float e, p, u[4], pi[10], Pi;
+  float nb, V[4];  // baryon extension (with ev.J)
+};
+
+// One workgroup's list of cells that the mixed path re-solves in FP64.
+struct LandauShared {
+  int n64;
+  int who[256];   // threads whose cell needs FP64
+  int tidx[256];  // T index of each listed thread's cell
+  float e[256], u[4][256];
+};
+
+// Landau frame of the workgroup's cells (one per thread, `live` marks real
+// cells; every thread of the workgroup calls this).  FP32 path: FP64 Jacobi for
+// every cell.  Mixed path: FP32 Jacobi for every cell, then the cells that
+// FP32 cannot decide are gathered into a dense list and re-solved in FP64 by
+// the first threads of the workgroup, so FP64 work is not spread over
+// partly-idle waves.
+__device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, const float t[10], float *e,
+                                   float u[4], LandauShared &S) {
+  if (!ev.mixed) {
+    if (live) landau_fp64(t, ev.tau, e, u);
+    return;
   }
+  bool need = false;
+  if (live) need = !landau_fp32_ok(t, ev.tau, e, u);
+  __syncthreads();  // the previous group's list is no longer read
+  if (threadIdx.x == 0) S.n64 = 0;
+  __syncthreads();
+  if (need) {
+    const int k = atomicAdd(&S.n64, 1);
+    S.who[k] = threadIdx.x;
+    S.tidx[threadIdx.x] = (int)tidx;
+  }
+  __syncthreads();
+  const int n = S.n64;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int w = S.who[k];
+    const long ti = S.tidx[w];
+    float tt[10], e64, u64[4];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) tt[c] = ev.T[c * ev.t_plane + ti];
+    landau_fp64(tt, ev.tau, &e64, u64);
+    S.e[w] = e64;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) S.u[c][w] = u64[c];
+  }
+  __syncthreads();
+  if (need) {
+    *e = S.e[threadIdx.x];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) u[c] = S.u[c][threadIdx.x];
+  }
+}
+
+// Landau frame, EoS, bulk and shear parts of the cell whose T components sit
+// at index tidx of ev.T's planes (workgroup-uniform call, see above).
+__device__ void landau_cell_compute(const EventLandau &ev, bool live, long tidx, CellOut &r, LandauShared &S) {
+#pragma clang fp contract(off)
   float t[10];
 #pragma unroll
-  for (int c = 0; c < 10; ++c) t[c] = ev.T[c * ev.t_plane + tidx];
-  float e, u[4];
-  if (ev.mixed)
-    landau_mixed(t, ev.tau, &e, u);
-  else
-    landau_fp64(t, ev.tau, &e, u);
+  for (int c = 0; c < 10; ++c) t[c] = live ? ev.T[c * ev.t_plane + tidx] : 0.0f;
+  float e = 0.0f, u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  landau_group_solve(ev, live, tidx, t, &e, u, S);
+  if (!live) return;
   const float TAU = ev.tau;
   const float p = eos_pressure(e, ev.eos_type);
-  float Pi = 0.0f, pi[10];
+  float Pi = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 10; ++c) pi[c] = 0.0f;
+  for (int c = 0; c < 10; ++c) r.pi[c] = 0.0f;
   if (ev.viscous) {
+    // src/LandauMatch.cpp:332-367
     const float a = t[0] - t[4] - t[7] - TAU * TAU * t[9];
     Pi = (float)((-1.0 / 3.0) * (double)(a - e) - (double)p);
     const float b = e + p + Pi;
     const float c = p + Pi;
-    pi[0] = t[0] - u[0] * u[0] * b + c;
-    pi[1] = t[1] - u[0] * u[1] * b;
-    pi[2] = t[2] - u[0] * u[2] * b;
-    pi[3] = t[3] - u[0] * u[3] * b;
-    pi[4] = t[4] - u[1] * u[1] * b - c;
-    pi[5] = t[5] - u[1] * u[2] * b;
-    pi[6] = t[6] - u[1] * u[3] * b;
-    pi[7] = t[7] - u[2] * u[2] * b - c;
-    pi[8] = t[8] - u[2] * u[3] * b;
-    pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * (1.0 / (double)(TAU * TAU)));
+    r.pi[0] = t[0] - u[0] * u[0] * b + c;
+    r.pi[1] = t[1] - u[0] * u[1] * b;
+    r.pi[2] = t[2] - u[0] * u[2] * b;
+    r.pi[3] = t[3] - u[0] * u[3] * b;
+    r.pi[4] = t[4] - u[1] * u[1] * b - c;
+    r.pi[5] = t[5] - u[1] * u[2] * b;
+    r.pi[6] = t[6] - u[1] * u[3] * b;
+    r.pi[7] = t[7] - u[2] * u[2] * b - c;
+    r.pi[8] = t[8] - u[2] * u[3] * b;
+    r.pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * (1.0 / (double)(TAU * TAU)));
   }
-  if (ev.out) {
-    const long P = ev.out_plane;
-    double *o = ev.out;
-    if (ev.fill_outputs) {
-      const double hb = (double)FSM_HBARC_F;
-      o[0 * P + is] = (double)e * hb;
-      o[1 * P + is] = (double)p * hb;
-      o[2 * P + is] = (double)u[0];
-      o[3 * P + is] = (double)u[1];
-      o[4 * P + is] = (double)u[2];
-      o[5 * P + is] = (double)u[3];
+  r.e = e;
+  r.p = p;
+  r.Pi = Pi;
 #pragma unroll
-      for (int c = 0; c < 10; ++c) o[(6 + c) * P + is] = (double)pi[c] * hb;
-      o[16 * P + is] = (double)Pi * hb;
-    } else {
-#pragma unroll
-      for (int c = 0; c < 17; ++c) o[c * P + is] = 0.0;
-    }
-  }
+  for (int c = 0; c < 4; ++c) r.u[c] = u[c];
   if (ev.J) {
     // src/LandauMatch.cpp:370-392: n_B = u_mu J^mu, V^mu = J^mu - n_B u^mu
     float j[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) j[c] = ev.J[c * ev.t_plane + tidx];
     const float nb = (u[0] * j[0]) - (u[1] * j[1]) - (u[2] * j[2]) - (TAU * TAU * u[3] * j[3]);
-    const long P = ev.out_plane;
-    ev.out_b[is] = (double)nb;
+    r.nb = nb;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) ev.out_b[(1 + c) * P + is] = (double)(j[c] - (nb * u[c]));
+    for (int c = 0; c < 4; ++c) r.V[c] = j[c] - (nb * u[c]);
   }
+}
+
+// Output plane c (0..21): the float value and whether the reference scales it
+// by hbarc when it fills its double vectors (src/FreestreamMilne.cpp:695-718):
+// the 17 vectors in output_to_vectors order, then n_B, V^mu.
+__device__ __forceinline__ float cell_value(const CellOut &r, int c) {
+  if (c == 0) return r.e;
+  if (c == 1) return r.p;
+  if (c < 6) return r.u[c - 2];
+  if (c < 16) return r.pi[c - 6];
+  if (c == 16) return r.Pi;
+  if (c == FSM_NUM_OUTPUTS) return r.nb;
+  return r.V[c - FSM_NUM_OUTPUTS - 1];
+}
+__device__ __forceinline__ bool plane_scaled(int c) { return c < 2 || (c >= 6 && c < FSM_NUM_OUTPUTS); }
+// (double)x * hbarc is exact in double (two 24-bit significands), so a value
+// staged as float and converted later gives the same double.
+__device__ __forceinline__ double plane_double(const EventLandau &ev, float x, int c) {
+  if (c < FSM_NUM_OUTPUTS && !ev.fill_outputs) return 0.0;
+  return plane_scaled(c) ? (double)x * (double)FSM_HBARC_F : (double)x;
+}
+
+// Float outputs of the function-level boundary (fsm_landau_device).
+__device__ void cell_store_float(const EventLandau &ev, long is, const CellOut &r) {
   const long D = ev.dim;
-  if (ev.fe) ev.fe[is] = e;
+  if (ev.fe) ev.fe[is] = r.e;
   if (ev.fu) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) ev.fu[c * D + is] = u[c];
+    for (int c = 0; c < 4; ++c) ev.fu[c * D + is] = r.u[c];
   }
-  if (ev.fp) ev.fp[is] = p;
-  if (ev.fPi) ev.fPi[is] = Pi;
+  if (ev.fp) ev.fp[is] = r.p;
+  if (ev.fPi) ev.fPi[is] = r.Pi;
   if (ev.fpi) {
 #pragma unroll
-    for (int c = 0; c < 10; ++c) ev.fpi[c * D + is] = pi[c];
+    for (int c = 0; c < 10; ++c) ev.fpi[c * D + is] = r.pi[c];
+  }
+}
+
+// One cell per thread, reference index is (T read through the layout map);
+// workgroup-uniform call.
+__device__ void landau_cell_out(const EventLandau &ev, bool live, long is, LandauShared &S) {
+  long tidx = is;
+  if (ev.t_eta_major) {
+    const long xy = is / ev.neta;
+    const long ie = is - xy * ev.neta;
+    tidx = ie * ev.nxy + xy;
+  }
+  CellOut r;
+  landau_cell_compute(ev, live, tidx, r, S);
+  if (!live) return;
+  if (ev.out) {
+#pragma unroll
+    for (int c = 0; c < FSM_NUM_OUTPUTS; ++c) ev.out[c * ev.out_plane + is] = plane_double(ev, cell_value(r, c), c);
+  }
+  if (ev.J) {
+#pragma unroll
+    for (int c = FSM_NUM_OUTPUTS; c < FSM_NUM_OUTPUTS + 5; ++c)
+      ev.out_b[(c - FSM_NUM_OUTPUTS) * ev.out_plane + is] = plane_double(ev, cell_value(r, c), c);
+  }
+  cell_store_float(ev, is, r);
+}
+
+// 3-D events (T eta-major): a tile of FSML_TE eta planes x FSML_TX transverse
+// cells.  T is read along the transverse index (coalesced in the eta-major
+// layout); the outputs go through LDS and leave along eta, the fast index of
+// the reference layout, so both sides move whole cache-line runs.
+__device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds, LandauShared &S) {
+  const int xl = threadIdx.x % FSML_TX, el = threadIdx.x / FSML_TX;
+  const long xy = (long)xt * FSML_TX + xl;
+  const int ie = et * FSML_TE + el;
+  const bool live = xy < ev.nxy && ie < ev.neta;
+  const int np = ev.J ? FSM_NUM_OUTPUTS + 5 : FSM_NUM_OUTPUTS;
+  CellOut r;
+  landau_cell_compute(ev, live, (long)ie * ev.nxy + xy, r, S);  // starts with a barrier
+  if (live) {
+    const int li = xl * FSML_PITCH + el;
+    for (int c = 0; c < np; ++c) lds[c * FSML_PLANE + li] = cell_value(r, c);
+    cell_store_float(ev, xy * ev.neta + ie, r);
+  }
+  __syncthreads();
+  const int wxl = threadIdx.x / FSML_TE, wel = threadIdx.x % FSML_TE;
+  const long wxy = (long)xt * FSML_TX + wxl;
+  const int wie = et * FSML_TE + wel;
+  if (wxy < ev.nxy && wie < ev.neta) {
+    const long is = wxy * ev.neta + wie;
+    const int li = wxl * FSML_PITCH + wel;
+    for (int c = (ev.out ? 0 : FSM_NUM_OUTPUTS); c < np; ++c) {
+      double *o = c < FSM_NUM_OUTPUTS ? ev.out + c * ev.out_plane : ev.out_b + (c - FSM_NUM_OUTPUTS) * ev.out_plane;
+      o[is] = plane_double(ev, lds[c * FSML_PLANE + li], c);
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
+  __shared__ LandauShared S;
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long is = (long)blockIdx.x * blockDim.x + threadIdx.x; is < ev.dim; is += stride)
-    landau_cell_out(ev, is);
+  for (long base = (long)blockIdx.x * blockDim.x; base < ev.dim; base += stride) {
+    const long is = base + threadIdx.x;
+    landau_cell_out(ev, is < ev.dim, is, S);
+  }
 }
 
-// Batched: item = (event, first cell, cell count).
+__global__ __launch_bounds__(256) void k_landau3d_one(EventLandau ev) {
+  extern __shared__ float lds_out[];
+  __shared__ LandauShared S;
+  const int xtiles = (ev.nxy + FSML_TX - 1) / FSML_TX;
+  const int b = xcd_order(blockIdx.x, gridDim.x);
+  landau3d_tile(ev, b % xtiles, b / xtiles, lds_out, S);
+}
+
+// Batched: item = (event, cell offset, cell count, 0) for 2-D events, or
+// (event, transverse tile, eta tile, 1) for 3-D events (eta-major T).
 __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restrict__ evs,
                                                       const int4 *__restrict__ items, int n_items) {
+  extern __shared__ float lds_out[];
+  __shared__ LandauShared S;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     int4 item = items[xcd_order(it, n_items)];
     const EventLandau ev = evs[item.x];
-    for (int c = threadIdx.x; c < item.z; c += blockDim.x) landau_cell_out(ev, (long)item.y + c);
+    if (item.w) {
+      landau3d_tile(ev, item.y, item.z, lds_out, S);
+    } else {
+      for (int base = 0; base < item.z; base += blockDim.x)
+        landau_cell_out(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
+    }
   }
 }
 
@@ -776,6 +917,12 @@ hipError_t fsm_launch_transpose(const float *in, float *out, int rows, int cols,
 }
 
 hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s) {
+  if (ev.t_eta_major && ev.neta > 1) {
+    const int xtiles = (ev.nxy + FSML_TX - 1) / FSML_TX, etiles = (ev.neta + FSML_TE - 1) / FSML_TE;
+    hipLaunchKernelGGL(k_landau3d_one, dim3(xtiles * etiles), dim3(256), fsm_landau_lds_bytes(ev.J != nullptr), s,
+                       ev);
+    return hipGetLastError();
+  }
   int blocks = (int)std::min<long>((ev.dim + 255) / 256, 65535L);
   hipLaunchKernelGGL(k_landau_one, dim3(blocks), dim3(256), 0, s, ev);
   return hipGetLastError();
@@ -818,8 +965,13 @@ hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int 
   return hipGetLastError();
 }
 
-hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s) {
+size_t fsm_landau_lds_bytes(int baryon) {
+  return (size_t)(baryon ? FSM_NUM_OUTPUTS + 5 : FSM_NUM_OUTPUTS) * FSML_PLANE * sizeof(float);
+}
+
+hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, size_t lds_bytes,
+                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_landau_batch, dim3(batch_grid(n)), dim3(256), 0, s, evs, items, n);
+  hipLaunchKernelGGL(k_landau_batch, dim3(batch_grid(n)), dim3(256), lds_bytes, s, evs, items, n);
   return hipGetLastError();
 }
