@@ -127,15 +127,25 @@ def timed(fn, steps, sync):
     return time.perf_counter() - t0
 
 
-def serve_run(fsm_amd, W, ctx, seconds, rate, pool, seed=5):
+def serve_run(fsm_amd, W, ctx, seconds, rate, pool, dev_pool=None, seed=5):
     """Poisson arrivals at `rate` per second for `seconds` drawn from the C4
     request pool (2-D and 3-D events, some with n_B) through the server;
-    latency = submit to results in HBM."""
+    latency = submit to results in HBM.  With dev_pool (the pool's inputs
+    already in HBM) requests are submitted with fsm_server_submit_device,
+    otherwise from host memory."""
     rng = np.random.default_rng(seed)
     srv = fsm_amd.Server(ctx, max_batch_events=128, max_inflight=2)
+
+    def submit(k):
+        p, e, nb = pool[k]
+        if dev_pool is None:
+            return srv.submit(p, e, nb)
+        de, dn = dev_pool[k]
+        return srv.submit_device(p, de.data_ptr(), e.size, None if dn is None else dn.data_ptr())
+
     # every shape once outside the timed run: plans, tile tuning, memory pools
-    for p, e, nb in pool:
-        rid = srv.submit(p, e, nb)
+    for k in range(len(pool)):
+        rid = submit(k)
         while srv.poll(100000) != rid:
             pass
         srv.release(rid)
@@ -146,8 +156,7 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, pool, seed=5):
     while True:
         now = time.perf_counter()
         if now - t0 < seconds and now >= t_next:
-            p, e, nb = pool[rng.integers(len(pool))]
-            rid = srv.submit(p, e, nb)
+            rid = submit(int(rng.integers(len(pool))))
             if rid is None:
                 busy += 1
             else:
@@ -168,7 +177,8 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, pool, seed=5):
     stats = srv.stats()
     srv.close()
     lat = np.array(lat) if lat else np.zeros(1)
-    return {"offered_per_s": rate, "completed": done, "refused_busy": busy,
+    return {"offered_per_s": rate, "inputs": "host" if dev_pool is None else "device",
+            "completed": done, "refused_busy": busy,
             "events_per_s": round(done / el, 2),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                            "p99": round(float(np.percentile(lat, 99)), 4)},
@@ -365,13 +375,18 @@ def main():
         pool = W.serving_mix(args.serve_pool, seed=11)
         n3 = sum(1 for p, _, _ in pool if p.DIM_ETA > 1)
         nb = sum(1 for _, _, b in pool if b is not None)
-        sweep = [serve_run(fsm_amd, W, ctx, args.serve_seconds, float(r), pool)
+        dev_pool = [(torch.from_numpy(e).to(dev), None if b is None else torch.from_numpy(b).to(dev))
+                    for _, e, b in pool]
+        sync()
+        sweep = [serve_run(fsm_amd, W, ctx, args.serve_seconds, float(r), pool, dev_pool)
                  for r in args.serve_rates.split(",")]
+        host_point = serve_run(fsm_amd, W, ctx, args.serve_seconds, float(args.serve_rates.split(",")[1]), pool)
+        del dev_pool
         serve = {"workload": "Poisson arrivals for %.1f s per rate from a pool of %d requests: %d 3-D (131^2 x "
                              "DIM_ETA 41-101, DIM_RAP 21), the rest C3-style 2-D; %d carry n_B (BARYON=1, "
-                             "SIGMA_B=0.8); host inputs, results kept in HBM; mixed path"
-                             % (args.serve_seconds, len(pool), n3, nb),
-                 "sweep": sweep}
+                             "SIGMA_B=0.8); profiles resident in HBM (submit_device), one rate also from host "
+                             "memory; results kept in HBM; mixed path" % (args.serve_seconds, len(pool), n3, nb),
+                 "sweep": sweep, "host_inputs": host_point}
 
     # ---- CPU baseline (oracle float path = the reference algorithm) --------------
     cpu = None

@@ -290,14 +290,24 @@ void fsm_server_destroy(fsm_server *s) {
   delete s;
 }
 
-int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV, const float *nB, size_t n) {
-  if (!s || !p || !e_GeV) return FSM_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Admission shared by the host- and device-input forms.  `e_host` (may be
+// NULL) is the host copy of the input used by fsm_params_finalize; the input
+// itself is copied by `upload` into the request's device buffer on the upload
+// stream.
+template <typename Upload>
+int64_t submit_impl(fsm_server *s, const fsm_params *p, const float *e_host, bool have_nb, size_t n,
+                    Upload upload) {
   fsm_params q = *p;
-  int st = fsm_params_finalize(&q, e_GeV, n);
+  int st = fsm_params_finalize(&q, e_host, n);
   if (st) return st;
+  if (!e_host && n != (size_t)q.DIM) return FSM_ERR_SIZE;
   // BARYON = 1 needs n_B; with BARYON = 0 an n_B is ignored, as the reference ignores charge then
-  if (q.BARYON && !nB) return FSM_ERR_ARG;
-  const bool baryon = q.BARYON && nB;
+  if (q.BARYON && !have_nb) return FSM_ERR_ARG;
+  const bool baryon = q.BARYON && have_nb;
   st = fsm_host::plan_check(s->ctx, &q);
   if (st) return st;
   Request *r = new (std::nothrow) Request();
@@ -328,16 +338,11 @@ int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV
     r->d_in = (float *)s->dpool.get(r->in_bytes);
     r->d_out = (double *)s->dpool.get(r->out_bytes);
     r->ws = s->dpool.get(r->ws_bytes);
-    r->h_in = (float *)s->hpool.get(r->in_bytes);
-    ok = r->d_in && r->d_out && r->ws && r->h_in;
+    r->h_in = e_host ? (float *)s->hpool.get(r->in_bytes) : nullptr;
+    ok = r->d_in && r->d_out && r->ws && (r->h_in || !e_host);
   }
   if (ok && !r->uploaded) ok = hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming) == hipSuccess;
-  if (ok) {
-    memcpy(r->h_in, e_GeV, (size_t)q.DIM * sizeof(float));
-    if (baryon) memcpy(r->h_in + q.DIM, nB, (size_t)q.DIM * sizeof(float));
-    ok = hipMemcpyAsync(r->d_in, r->h_in, r->in_bytes, hipMemcpyHostToDevice, s->upload_stream) == hipSuccess &&
-         hipEventRecord(r->uploaded, s->upload_stream) == hipSuccess;
-  }
+  if (ok) ok = upload(r, baryon) && hipEventRecord(r->uploaded, s->upload_stream) == hipSuccess;
   std::lock_guard<std::mutex> g(s->mu);
   if (!ok) {
     s->bytes_in_use -= r->footprint;
@@ -351,6 +356,55 @@ int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV
   ++s->n_admitted;
   s->cv_work.notify_one();
   return r->id;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV, const float *nB, size_t n) {
+  if (!s || !p || !e_GeV) return FSM_ERR_ARG;
+  if (p->BARYON && !nB) return FSM_ERR_ARG;
+  return submit_impl(s, p, e_GeV, nB != nullptr, n, [&](Request *r, bool baryon) {
+    const size_t D = (size_t)r->p.DIM;
+    memcpy(r->h_in, e_GeV, D * sizeof(float));
+    if (baryon) memcpy(r->h_in + D, nB, D * sizeof(float));
+    return hipMemcpyAsync(r->d_in, r->h_in, r->in_bytes, hipMemcpyHostToDevice, s->upload_stream) == hipSuccess;
+  });
+}
+
+int64_t fsm_server_submit_device(fsm_server *s, const fsm_params *p, const float *d_e_GeV, const float *d_nB,
+                                 size_t n, void *stream) {
+  if (!s || !p || !d_e_GeV) return FSM_ERR_ARG;
+  if (p->BARYON && !d_nB) return FSM_ERR_ARG;
+  hipStream_t cs = (hipStream_t)stream;
+  // the copies below run on the upload stream after the caller's stream
+  hipEvent_t ready = nullptr;
+  if (hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) return FSM_ERR_DEVICE;
+  if (hipEventRecord(ready, cs) != hipSuccess || hipStreamWaitEvent(s->upload_stream, ready, 0) != hipSuccess) {
+    (void)hipEventDestroy(ready);
+    return FSM_ERR_DEVICE;
+  }
+  (void)hipEventDestroy(ready);  // released once the recorded work completes
+  // E_DEP_FS needs the reference's serial float sums over the input
+  // (src/FreeStream.cpp:389-417): those run on a host copy.
+  std::vector<float> host;
+  if (p->E_DEP_FS) {
+    if (n != (size_t)p->DIM_X * p->DIM_Y * p->DIM_ETA) return FSM_ERR_SIZE;
+    host.resize(n);
+    if (hipStreamSynchronize(s->upload_stream) != hipSuccess ||
+        hipMemcpy(host.data(), d_e_GeV, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return FSM_ERR_DEVICE;
+  }
+  return submit_impl(s, p, host.empty() ? nullptr : host.data(), d_nB != nullptr, n, [&](Request *r, bool baryon) {
+    const size_t D = (size_t)r->p.DIM;
+    bool ok = hipMemcpyAsync(r->d_in, d_e_GeV, D * sizeof(float), hipMemcpyDeviceToDevice, s->upload_stream) ==
+              hipSuccess;
+    if (ok && baryon)
+      ok = hipMemcpyAsync(r->d_in + D, d_nB, D * sizeof(float), hipMemcpyDeviceToDevice, s->upload_stream) ==
+           hipSuccess;
+    return ok;
+  });
 }
 
 int64_t fsm_server_poll(fsm_server *s, int timeout_us) {

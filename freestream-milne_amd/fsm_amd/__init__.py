@@ -168,6 +168,8 @@ def _lib():
         lib.fsm_server_destroy.restype = None
         lib.fsm_server_submit.argtypes = [vp, P, fp, fp, ctypes.c_size_t]
         lib.fsm_server_submit.restype = ctypes.c_int64
+        lib.fsm_server_submit_device.argtypes = [vp, P, vp, vp, ctypes.c_size_t, vp]
+        lib.fsm_server_submit_device.restype = ctypes.c_int64
         lib.fsm_server_poll.argtypes = [vp, ctypes.c_int]
         lib.fsm_server_poll.restype = ctypes.c_int64
         lib.fsm_server_result.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
@@ -432,6 +434,18 @@ class Server:
         if rid < 0:
             raise FsmError(int(rid), "fsm_server_submit")
         self._dims[rid] = e.size
+        return rid
+
+    def submit_device(self, p: Params, d_e: int, n: int, d_nB: int = None, stream=None):
+        """Same with the input (and n_B) already in device memory (pointers);
+        the server copies them after the work queued on `stream`."""
+        q = p.copy()
+        rid = _lib().fsm_server_submit_device(self._s, ctypes.byref(q), d_e, d_nB, n, _stream_handle(stream))
+        if rid == FSM_ERR_BUSY:
+            return None
+        if rid < 0:
+            raise FsmError(int(rid), "fsm_server_submit_device")
+        self._dims[rid] = n
         return rid
 
     def poll(self, timeout_us: int = 0):

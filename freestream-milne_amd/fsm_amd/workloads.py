@@ -167,14 +167,21 @@ def serving_mix(n_events: int, seed: int = 11):
     requests carry a baryon density (BARYON = 1, SIGMA_B = 0.8)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     twod = batch_mix(n_events, seed=seed)
+    # exact proportions in the pool (rounded), in shuffled order
+    is3d = np.zeros(n_events, dtype=bool)
+    is3d[: int(round(0.2 * n_events))] = True
+    isb = np.zeros(n_events, dtype=bool)
+    isb[: int(round(0.25 * n_events))] = True
+    rng.shuffle(is3d)
+    rng.shuffle(isb)
     out = []
     for k in range(n_events):
-        if rng.uniform() < 0.2:
+        if is3d[k]:
             p, e = c2_like(int(rng.integers(41, 102)), seed=seed * 1009 + k)
         else:
             p, e = twod[k]
         nb = None
-        if rng.uniform() < 0.25:
+        if isb[k]:
             p = with_baryon(p)
             nb = baryon_density(e)
         out.append((p, e, nb))

@@ -286,6 +286,16 @@ void fsm_server_destroy(fsm_server *s);
  * slots or the HBM budget are exhausted (nothing admitted), or another
  * negative status. */
 int64_t fsm_server_submit(fsm_server *s, const fsm_params *p, const float *e_GeV, const float *nB, size_t n);
+/* Same with the inputs already in device memory (d_e_GeV, d_nB: n floats
+ * each; d_nB only for BARYON = 1): the server copies them into its own
+ * buffers on its upload stream, after the work already queued on `stream`
+ * (a hipStream_t; NULL = default stream), so the caller may write its
+ * buffers again once that copy is done (at the latest when the request is
+ * reported by poll).  With E_DEP_FS = 1 the input is also copied to the host
+ * for the reference's serial float sums (src/FreeStream.cpp:389-417).  No
+ * reference counterpart: keeps event profiles in HBM end to end. */
+int64_t fsm_server_submit_device(fsm_server *s, const fsm_params *p, const float *d_e_GeV, const float *d_nB,
+                                 size_t n, void *stream);
 /* Id of a completed request not yet reported by poll, or -1 if none completes
  * within timeout_us microseconds (0: do not wait). */
 int64_t fsm_server_poll(fsm_server *s, int timeout_us);

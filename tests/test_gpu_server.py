@@ -116,3 +116,31 @@ def test_budget_admission_and_cancel(gpu_ctx):
     srv.release(a)
     assert srv.stats()["bytes_in_use"] == 0
     srv.close()
+
+
+def test_device_inputs_match_host_inputs(gpu_ctx):
+    """fsm_server_submit_device (profiles already in HBM) gives the same
+    results as the host-input submit, for 2-D, 3-D and baryon requests."""
+    import torch
+
+    pool = W.serving_mix(6, seed=3)
+    pool.append((W.with_baryon(W.small_3d()[0]), W.small_3d()[1], W.baryon_density(W.small_3d()[1])))
+    srv = fsm_amd.Server(gpu_ctx, max_batch_events=4, max_inflight=2)
+    s = torch.cuda.current_stream()
+    for p, e, nb in pool:
+        rid = srv.submit(p, e, nb)
+        assert srv.poll(5000000) == rid
+        ref = srv.fetch(rid) if nb is None else srv.fetch_baryon(rid)
+        de = torch.from_numpy(e).cuda()
+        dn = None if nb is None else torch.from_numpy(nb).cuda()
+        rid = srv.submit_device(p, de.data_ptr(), e.size, None if dn is None else dn.data_ptr(), stream=s)
+        assert srv.poll(5000000) == rid
+        got = srv.fetch(rid) if nb is None else srv.fetch_baryon(rid)
+        if nb is None:
+            _same(got, ref, 0)
+        else:
+            _same(got[0], ref[0], 0)
+            assert np.array_equal(got[1], ref[1])
+            for a, b in zip(got[2], ref[2]):
+                assert np.array_equal(a, b)
+    srv.close()
