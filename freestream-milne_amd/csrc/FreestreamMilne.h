@@ -9,12 +9,16 @@
 //   float tau = fs.tau_LandauMatch;
 //
 // Behaviour kept from the reference: a missing ./freestream_input prints a
-// message and uses the built-in defaults (src/FileIO.cpp:178-181); a vector
-// whose size differs from DIM prints the two sizes and calls exit(-1)
-// (src/FreestreamMilne.cpp:259-264); an initial condition other than the
-// vector path prints "Not a valid initial Condition... Goodbye" and returns 0
-// without outputs (src/FreestreamMilne.cpp:277-281 -- the file and analytic
-// initial conditions are not part of this library).
+// message and uses the built-in defaults (src/FileIO.cpp:178-181); the
+// initial condition is dispatched on IC_ENERGY / IC_BARYON as in
+// src/FreestreamMilne.cpp:231-298 (5 = the vector given to
+// initialize_from_vector, the others from the reference's generators and
+// ./initial_profiles files); a vector whose size differs from DIM prints the
+// two sizes and calls exit(-1) (:259-264); an unknown initial condition
+// prints "Not a valid initial Condition... Goodbye" and returns 0 (:277-281);
+// a missing profile file prints the reference's message and calls exit(1)
+// (src/FileIO.cpp:150-154); the output/*.dat files are written into ./output
+// when that directory exists, and the reference's summary lines are printed.
 #ifndef FSM_FREESTREAMMILNE_H_
 #define FSM_FREESTREAMMILNE_H_
 
@@ -40,15 +44,18 @@ class FREESTREAMMILNE {
     if (fsm_params_read(&p, "freestream_input") == FSM_ERR_FILE)
       printf("Couldn't open parameters.dat . Using default values!\n");
     p.DIM = p.DIM_X * p.DIM_Y * p.DIM_ETA;
-    if (p.IC_ENERGY != 5) {
-      printf("Not a valid initial Condition... Goodbye\n");
+    std::vector<float> e_in, nb_in;
+    if (p.IC_ENERGY == 5) {
+      if ((size_t)p.DIM != init_energy_density.size()) {
+        printf("Grid dimension of input vector does not match freestream_input file! \n");
+        printf("Vector size : %d , DIM = %d \n", (int)init_energy_density.size(), p.DIM);
+        exit(-1);
+      }
+      e_in = init_energy_density;
+    } else if (!make_ic(p, FSM_IC_WHICH_ENERGY, e_in)) {
       return 0;
     }
-    if ((size_t)p.DIM != init_energy_density.size()) {
-      printf("Grid dimension of input vector does not match freestream_input file! \n");
-      printf("Vector size : %d , DIM = %d \n", (int)init_energy_density.size(), p.DIM);
-      exit(-1);
-    }
+    if (p.BARYON && !make_ic(p, FSM_IC_WHICH_BARYON, nb_in)) return 0;
     if (!ctx_) {
       int st = fsm_create(device_, &ctx_);
       if (st != FSM_OK) {
@@ -65,11 +72,20 @@ class FREESTREAMMILNE {
       outs[k]->assign((size_t)p.DIM, 0.0);
       ptrs[k] = outs[k]->data();
     }
-    int st = fsm_run(ctx_, &p, init_energy_density.data(), init_energy_density.size(), ptrs, &tau_LandauMatch);
+    fsm_run_report rep;
+    int st = fsm_run_files(ctx_, &p, e_in.data(), p.BARYON ? nb_in.data() : nullptr, e_in.size(), ptrs, "output",
+                           &rep);
     if (st != FSM_OK) {
       printf("run_freestream_milne: %s\n", fsm_status_string(st));
       exit(-1);
     }
+    tau_LandauMatch = rep.tau_landau_match;
+    printf("Total energy on grid before streaming : %f \n", rep.total_energy_before);
+    printf("Total energy on grid after streaming : %f \n", rep.total_energy_after);
+    printf("Fraction of energy contained in Freezeout Hypersurface : %f \n",
+           rep.energy_inside / rep.total_energy_after);
+    printf("R_Pi_Inv at center : %f \n", rep.R_Pi_Inv_center);
+    printf("R_pimunu_Inv at center : %f \n", rep.R_pimunu_Inv_center);
     return 0;
   }
 
@@ -128,6 +144,22 @@ class FREESTREAMMILNE {
   std::vector<double> final_Pi;
 
  private:
+  // src/FreestreamMilne.cpp:231-298 for one density; false (after the
+  // reference's message) when the code is not a valid initial condition
+  bool make_ic(fsm_params &p, int which, std::vector<float> &v) {
+    v.assign((size_t)p.DIM, 0.0f);
+    const int st = fsm_initial_condition(&p, which, "initial_profiles", v.data(), v.size());
+    if (st == FSM_ERR_FILE) {
+      printf("Couldn't open initial profile!\n");
+      exit(1);
+    }
+    if (st != FSM_OK) {
+      printf("Not a valid initial Condition... Goodbye\n");
+      return false;
+    }
+    return true;
+  }
+
   fsm_ctx *ctx_;
   int device_;
 };

@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -494,6 +496,7 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.mscale = pl.mscale.p;
   pl.charge_norm = charge_norm_factor(p.SIGMA_B);
   ev.charge = 0;
+  ev.raw_e = p.IC_ENERGY != 5;
   ev.in_norm = pl.charge_norm;
   ev.out_j = 0;
   ev.jscale = p.DRAP;
@@ -720,6 +723,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     ev.norm_factor = (float)(1.0 / (2.0 * M_PI * (double)nf));
   }
   ev.charge = 0;
+  ev.raw_e = p.IC_ENERGY != 5;
   if (p.BARYON) {
     // charge pass: same geometry; the density carries the charge rapidity
     // factor on the uniform DRAP grid (src/FreeStream.cpp:368-385) and
@@ -1207,6 +1211,178 @@ int fsm_run_baryon(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, const float 
         FSM_TRY(hipMemcpyAsync(V_out[c], ctx->d_out.p + (FSM_NUM_OUTPUTS + 1 + c) * D, D * sizeof(double),
                                hipMemcpyDeviceToHost, s));
   FSM_TRY(hipStreamSynchronize(s));
+  return FSM_OK;
+}
+
+int fsm_diagnostics_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e, const float *d_u, const float *d_p,
+                           const float *d_Pi, const float *d_pi, const fsm_diag_out *out, void *stream) {
+  if (!ctx || !p || !out) return FSM_ERR_ARG;
+  bool pdu = false, bw = false;
+  for (int c = 0; c < 4; ++c) pdu |= out->pi_dot_u[c] != nullptr;
+  for (int c = 0; c < 4; ++c) bw |= out->beta[c] != nullptr;
+  for (int c = 0; c < 6; ++c) bw |= out->omega[c] != nullptr;
+  // every requested output needs its inputs
+  if ((out->R_Pi_Inv && (!d_p || !d_Pi)) || (out->R_pimunu_Inv && (!d_pi || !d_e || !d_p)) ||
+      (pdu && (!d_u || !d_pi)) || (out->pi_mu_mu && !d_pi) || (bw && (!d_e || !d_u)))
+    return FSM_ERR_ARG;
+  FSM_TRY(hipSetDevice(ctx->device));
+  FSM_TRY(fsm_launch_diagnostics(*p, d_e, d_u, d_p, d_Pi, d_pi, *out, (hipStream_t)stream));
+  return FSM_OK;
+}
+
+int fsm_run_files(fsm_ctx *ctx, fsm_params *p, const float *e_in, const float *nB_in, size_t n,
+                  double *const out[FSM_NUM_OUTPUTS], const char *output_dir, fsm_run_report *rep) {
+#pragma clang fp contract(off)
+  if (!ctx || !p || !e_in) return FSM_ERR_ARG;
+  const bool baryon = p->BARYON != 0;
+  if (baryon && !nB_in) return FSM_ERR_ARG;
+  int st = fsm_params_finalize(p, e_in, n);
+  if (st) return st;
+  const size_t D = (size_t)p->DIM;
+  const float hbarc = 0.197326938f;
+  // the density the reference streams, fm^-4 (src/FreestreamMilne.cpp:267 for the vector input)
+  std::vector<float> e0(D);
+  for (size_t is = 0; is < D; ++is) e0[is] = p->IC_ENERGY == 5 ? e_in[is] * 1.0f / hbarc + 1.0e-7f : e_in[is];
+  // the step itself, with the values filled whatever OUTPUTFORMAT is (the
+  // reference writes its files from its float arrays in every format)
+  fsm_params q = *p;
+  q.OUTPUTFORMAT = 2;
+  const int np = baryon ? FSM_NUM_OUTPUTS_BARYON : FSM_NUM_OUTPUTS;
+  FSM_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = nullptr;
+  DevBuf<float> d_in, d_nb, d_T, d_f;
+  DevBuf<double> d_out;
+  FSM_TRY(d_in.ensure(D));
+  FSM_TRY(d_out.ensure(D * np));
+  FSM_TRY(d_T.ensure(10 * D));
+  FSM_TRY(hipMemcpyAsync(d_in.p, e_in, D * sizeof(float), hipMemcpyHostToDevice, s));
+  if (baryon) {
+    FSM_TRY(d_nb.ensure(D));
+    FSM_TRY(hipMemcpyAsync(d_nb.p, nB_in, D * sizeof(float), hipMemcpyHostToDevice, s));
+    st = fsm_run_baryon_device(ctx, &q, d_in.p, d_nb.p, d_out.p, s);
+  } else {
+    st = fsm_run_device(ctx, &q, d_in.p, d_out.p, s);
+  }
+  if (st) return st;
+  st = fsm_stream_tmunu_device(ctx, &q, d_in.p, d_T.p, s);
+  if (st) return st;
+  std::vector<double> h((size_t)np * D);
+  std::vector<float> T00(D);
+  FSM_TRY(hipMemcpyAsync(h.data(), d_out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSM_TRY(hipMemcpyAsync(T00.data(), d_T.p, D * sizeof(float), hipMemcpyDeviceToHost, s));
+  FSM_TRY(hipStreamSynchronize(s));
+  // float fields back from the doubles: (double)x * hbarc is exact, so the
+  // division returns x; e, p, u[4], pi[10], Pi (+ n_B, V[4]) in planes f[0..]
+  std::vector<float> f((size_t)np * D);
+  for (int c = 0; c < np; ++c) {
+    const bool scaled = c < 2 || (c >= 6 && c < FSM_NUM_OUTPUTS);
+    for (size_t is = 0; is < D; ++is)
+      f[c * D + is] = scaled ? (float)(h[c * D + is] / (double)hbarc) : (float)h[c * D + is];
+  }
+  const float *fe = &f[0], *fp = &f[D], *fu = &f[2 * D], *fpi = &f[6 * D], *fPi = &f[16 * D];
+  // diagnostics on the device from the float fields
+  const int nd = 1 + 1 + 4 + 1 + 4 + 6;
+  FSM_TRY(d_f.ensure(17 * D + nd * D));
+  FSM_TRY(hipMemcpyAsync(d_f.p, f.data(), 17 * D * sizeof(float), hipMemcpyHostToDevice, s));
+  float *dd = d_f.p + 17 * D;
+  fsm_diag_out o;
+  o.R_Pi_Inv = dd;
+  o.R_pimunu_Inv = dd + D;
+  for (int c = 0; c < 4; ++c) o.pi_dot_u[c] = dd + (2 + c) * D;
+  o.pi_mu_mu = dd + 6 * D;
+  for (int c = 0; c < 4; ++c) o.beta[c] = dd + (7 + c) * D;
+  for (int c = 0; c < 6; ++c) o.omega[c] = dd + (11 + c) * D;
+  st = fsm_diagnostics_device(ctx, &q, d_f.p, d_f.p + 2 * D, d_f.p + D, d_f.p + 16 * D, d_f.p + 6 * D, &o, s);
+  if (st) return st;
+  std::vector<float> diag((size_t)nd * D);
+  FSM_TRY(hipMemcpyAsync(diag.data(), dd, diag.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  FSM_TRY(hipStreamSynchronize(s));
+  // report (the values the reference prints)
+  float e_before = 0.0f;
+  for (size_t is = 0; is < D; ++is) e_before += e0[is];
+  if (p->DIM_ETA > 1)
+    e_before *= (p->TAU0 * p->DX * p->DY * p->DETA);
+  else
+    e_before *= (p->DX * p->DY);
+  float total = 0.0f, inside = 0.0f;
+  fsm_energy_sums(p, T00.data(), fe, &total, &inside);
+  const int NX = p->DIM_X, NY = p->DIM_Y, NE = p->DIM_ETA;
+  const size_t ctr = (size_t)(NY * NE) * ((NX - 1) / 2) + (size_t)NE * ((NY - 1) / 2) + (NE - 1) / 2;
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->total_energy_before = e_before;
+    rep->total_energy_after = total;
+    rep->energy_inside = inside;
+    rep->R_Pi_Inv_center = diag[ctr];
+    rep->R_pimunu_Inv_center = diag[D + ctr];
+    rep->tau_landau_match = p->TAU;
+    rep->dtau = p->DTAU;
+    rep->dim_phip = p->DIM_PHIP;
+  }
+  // host vectors: the 17 outputs, zero unless OUTPUTFORMAT is 2 or 3 (src/FreestreamMilne.cpp:695)
+  if (out) {
+    const bool fill = p->OUTPUTFORMAT == 2 || p->OUTPUTFORMAT == 3;
+    for (int c = 0; c < FSM_NUM_OUTPUTS; ++c)
+      if (out[c]) {
+        if (fill)
+          memcpy(out[c], &h[c * D], D * sizeof(double));
+        else
+          memset(out[c], 0, D * sizeof(double));
+      }
+  }
+  // the reference's output/*.dat files, in its order (src/FreestreamMilne.cpp:310-313,
+  // :555-559, :565-673); a missing directory writes nothing, as the
+  // reference's unopened streams do
+  if (output_dir) {
+    struct stat sb;
+    if (stat(output_dir, &sb) != 0 || !S_ISDIR(sb.st_mode)) return FSM_OK;
+    auto path = [&](const char *name) {
+      std::string s = output_dir;
+      if (!s.empty() && s.back() != '/') s += '/';
+      return s + name + ".dat";
+    };
+    auto scalar = [&](const char *name, const float *v) { fsm_write_scalar(p, path(name).c_str(), v); };
+    auto proj = [&](const char *name, const float *v) { fsm_write_projection(p, path(name).c_str(), v); };
+    std::vector<float> nb0;
+    if (baryon) nb0.assign(nB_in, nB_in + D);
+    scalar("initial_e", e0.data());
+    if (baryon) scalar("initial_nB", nb0.data());
+    proj("initial_e_projection", e0.data());
+    if (baryon) proj("initial_nB_projection", nb0.data());
+    fsm_write_energy_inside(path("energy_inside_switch_surf").c_str(), inside);
+    proj("R_Pi_Inv_projection", &diag[0]);
+    proj("R_pimunu_Inv_projection", &diag[D]);
+    static const char *pdu[4] = {"pi_dot_u_tau_projection", "pi_dot_u_x_projection", "pi_dot_u_y_projection",
+                                 "pi_dot_u_eta_projection"};
+    for (int c = 0; c < 4; ++c) proj(pdu[c], &diag[(2 + c) * D]);
+    proj("pi_mu_mu_projection", &diag[6 * D]);
+    scalar("e", fe);
+    scalar("p", fp);
+    scalar("bulk_PI", fPi);
+    proj("e_projection", fe);
+    proj("p_projection", fp);
+    proj("bulk_PI_projection", fPi);
+    static const char *un[4] = {"ut", "ux", "uy", "un"};
+    for (int c = 0; c < 4; ++c) scalar(un[c], fu + c * D);
+    for (int c = 0; c < 4; ++c) proj((std::string(un[c]) + "_projection").c_str(), fu + c * D);
+    static const char *pn[10] = {"pitt", "pitx", "pity", "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"};
+    for (int c = 0; c < 10; ++c) scalar(pn[c], fpi + c * D);
+    for (int c = 0; c < 10; ++c) proj((std::string(pn[c]) + "_projection").c_str(), fpi + c * D);
+    proj("beta_x_projection", &diag[8 * D]);
+    scalar("beta_x", &diag[8 * D]);
+    proj("beta_y_projection", &diag[9 * D]);
+    scalar("beta_y", &diag[9 * D]);
+    proj("w_xy_projection", &diag[14 * D]);
+    scalar("w_xy", &diag[14 * D]);
+    if (baryon) {
+      const float *fnb = &f[FSM_NUM_OUTPUTS * D], *fV = &f[(FSM_NUM_OUTPUTS + 1) * D];
+      scalar("nB", fnb);
+      proj("nB_projection", fnb);
+      scalar("V_x", fV + D);
+      scalar("V_y", fV + 2 * D);
+      scalar("V_eta", fV + 3 * D);
+    }
+  }
   return FSM_OK;
 }
 

@@ -131,6 +131,7 @@ struct Event2D {
   // ---- charge pass of the baryon extension ----
   int charge;           // 1: the input is n_B (fm^-3) and z = n_B * in_norm (convertInitialChargeDensity)
   float in_norm;        // its normalisation (src/FreeStream.cpp:365-366)
+  int raw_e;            // 1: the energy input is already the fm^-4 density (IC_ENERGY != 5)
   int out_j;            // 1: write J^tau, J^x, J^y, J^eta (4 planes of T) instead of T^{mu nu}
   float jscale;         // DRAP of calculateBaryonCurrent (src/LandauMatch.cpp:140)
 };
@@ -217,6 +218,7 @@ struct Event3D {
   long t_plane;
   float norm_factor;    // energy: convertInitialDensity norm; charge pass: convertInitialChargeDensity norm
   int charge;           // 1: the input is n_B: src = n_B * norm_factor (no hbarc, no floor)
+  int raw_e;            // 1: the energy input is already the fm^-4 density (IC_ENERGY != 5)
 };
 
 // --------------------------------------------------------------- Landau ----

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/fsm.h"
 #include "fsm_internal.h"
 
 hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s);
@@ -40,4 +41,7 @@ hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int 
 size_t fsm_landau_lds_bytes(int baryon);
 hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, size_t lds_bytes,
                                    hipStream_t s);
+// Diagnostics (fsm_diag.hip).
+hipError_t fsm_launch_diagnostics(const fsm_params &p, const float *e, const float *u, const float *pr,
+                                  const float *Pi, const float *pi, const fsm_diag_out &out, hipStream_t s);
 #endif  // FSM_KERNELS_H_

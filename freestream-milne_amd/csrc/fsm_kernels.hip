@@ -43,7 +43,7 @@ __device__ __forceinline__ float ingest_e0(float v) {
 // sum over rapidity copies is folded into the output scale).
 __device__ __forceinline__ float knot_z(const Event2D &ev, float v) {
   if (ev.charge) return v * ev.in_norm;
-  return (float)((double)ingest_e0(v) / (2.0 * M_PI));
+  return (float)((double)(ev.raw_e ? v : ingest_e0(v)) / (2.0 * M_PI));
 }
 
 // ======================================================================
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restric
 // e_gev (reference layout [nxy][neta]) -> src (eta-major [neta][nxy]) with
 // src = e0 * norm_factor (src/FreeStream.cpp:346, first product).
 __device__ void src3d_tile(const float *__restrict__ e_gev, float *__restrict__ src, int nxy, int neta,
-                           float norm_factor, int charge, int bx, int by, float (*tile)[65]) {
+                           float norm_factor, int raw, int bx, int by, float (*tile)[65]) {
   const int xy0 = bx * 64, et0 = by * 64;
   const int tx = threadIdx.x & 63, tyb = threadIdx.x >> 6;
   __syncthreads();
@@ -268,8 +268,9 @@ __device__ void src3d_tile(const float *__restrict__ e_gev, float *__restrict__ 
     if (xy < nxy && et < neta) {
       const float x = e_gev[(long)xy * neta + et];
       // src/FreeStream.cpp:384 (n_B * norm) or :346 (e0 * norm); the
-      // rapidity factor is applied per slot
-      v = (charge ? x : ingest_e0(x)) * norm_factor;
+      // rapidity factor is applied per slot.  `raw`: the input is already
+      // the density (n_B, or an fm^-4 energy profile)
+      v = (raw ? x : ingest_e0(x)) * norm_factor;
     }
     tile[r][tx] = v;
   }
@@ -281,9 +282,9 @@ __device__ void src3d_tile(const float *__restrict__ e_gev, float *__restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_src3d(const float *__restrict__ e_gev, float *__restrict__ src,
-                                               int nxy, int neta, float norm_factor, int charge) {
+                                               int nxy, int neta, float norm_factor, int raw) {
   __shared__ float tile[64][65];
-  src3d_tile(e_gev, src, nxy, neta, norm_factor, charge, blockIdx.x, blockIdx.y, tile);
+  src3d_tile(e_gev, src, nxy, neta, norm_factor, raw, blockIdx.x, blockIdx.y, tile);
 }
 
 // Batched: item = (event, xy tile, eta tile).
@@ -293,7 +294,8 @@ __global__ __launch_bounds__(256) void k_src3d_batch(const Event3D *__restrict__
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     int4 item = items[xcd_order(it, n_items)];
     const Event3D ev = evs[item.x];
-    src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, ev.charge, item.y, item.z, tile);
+    src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, ev.charge | ev.raw_e, item.y, item.z,
+               tile);
   }
 }
 
@@ -899,7 +901,8 @@ hipError_t fsm_launch_spline2d(const Event2D &ev, hipStream_t s) {
 hipError_t fsm_launch_src3d(const Event3D &ev, hipStream_t s) {
   int nxy = ev.nx * ev.ny;
   dim3 grid((nxy + 63) / 64, (ev.neta + 63) / 64);
-  hipLaunchKernelGGL(k_src3d, grid, dim3(256), 0, s, ev.e_gev, ev.src, nxy, ev.neta, ev.norm_factor, ev.charge);
+  hipLaunchKernelGGL(k_src3d, grid, dim3(256), 0, s, ev.e_gev, ev.src, nxy, ev.neta, ev.norm_factor,
+                     ev.charge | ev.raw_e);
   return hipGetLastError();
 }
 

@@ -92,7 +92,9 @@ int fsm_params_finalize(fsm_params *p, const float *e_GeV, size_t n) {
     const float hbarc = 0.197326938;
     float numerator = 0.0f, denominator = 0.0f;
     for (int is = 0; is < p->DIM; ++is) {
-      float eps = e_GeV[is] * 1.0f / hbarc + 1.0e-7f;  // src/FreestreamMilne.cpp:267
+      // the converted vector (src/FreestreamMilne.cpp:267), or the fm^-4
+      // profile of the other initial conditions as is
+      float eps = p->IC_ENERGY == 5 ? e_GeV[is] * 1.0f / hbarc + 1.0e-7f : e_GeV[is];
       numerator += (eps * eps);
       denominator += eps;
     }

@@ -47,6 +47,14 @@ FSM_ERR_CANCELLED = -9
 
 PATH_FP32 = 0
 PATH_MIXED = 1
+
+IC_WHICH_ENERGY = 0
+IC_WHICH_BARYON = 1
+PROFILE_XYEV = 0
+PROFILE_CPUVH = 1
+PROFILE_SUPERMC_BLOCK = 2
+PROFILE_TRENTO_BLOCK = 3
+PROFILE_TRENTO3D_BLOCK = 4
 TILES_AUTO = 0
 TILES_BATCH = 1
 
@@ -106,6 +114,24 @@ class ServerConfig(ctypes.Structure):
 
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_events", ctypes.c_int),
                 ("max_inflight", ctypes.c_int), ("max_requests", ctypes.c_int)]
+
+
+class RunReport(ctypes.Structure):
+    """fsm_run_report: the values run_freestream_milne prints."""
+    _fields_ = [("total_energy_before", ctypes.c_float), ("total_energy_after", ctypes.c_float),
+                ("energy_inside", ctypes.c_float), ("R_Pi_Inv_center", ctypes.c_float),
+                ("R_pimunu_Inv_center", ctypes.c_float), ("tau_landau_match", ctypes.c_float),
+                ("dtau", ctypes.c_float), ("dim_phip", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DiagOut(ctypes.Structure):
+    """fsm_diag_out: device pointers of the diagnostics (0 = not wanted)."""
+    _fields_ = [("R_Pi_Inv", ctypes.c_void_p), ("R_pimunu_Inv", ctypes.c_void_p),
+                ("pi_dot_u", ctypes.c_void_p * 4), ("pi_mu_mu", ctypes.c_void_p),
+                ("beta", ctypes.c_void_p * 4), ("omega", ctypes.c_void_p * 6)]
 
 
 class FsmError(RuntimeError):
@@ -181,6 +207,18 @@ def _lib():
         lib.fsm_server_cancel.argtypes = [vp, ctypes.c_int64]
         lib.fsm_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_size_t)]
+        lib.fsm_initial_condition.argtypes = [P, ctypes.c_int, ctypes.c_char_p, fp, ctypes.c_size_t]
+        lib.fsm_ic_seed.argtypes = [ctypes.c_uint]
+        lib.fsm_ic_seed.restype = None
+        lib.fsm_ic_rand.argtypes = []
+        lib.fsm_read_profile.argtypes = [P, ctypes.c_int, ctypes.c_char_p, fp, ctypes.c_size_t]
+        lib.fsm_write_scalar.argtypes = [P, ctypes.c_char_p, fp]
+        lib.fsm_write_projection.argtypes = [P, ctypes.c_char_p, fp]
+        lib.fsm_energy_sums.argtypes = [P, fp, fp, fp, fp]
+        lib.fsm_write_energy_inside.argtypes = [ctypes.c_char_p, ctypes.c_float]
+        lib.fsm_diagnostics_device.argtypes = [vp, P, vp, vp, vp, vp, vp, ctypes.POINTER(DiagOut), vp]
+        lib.fsm_run_files.argtypes = [vp, P, fp, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.c_char_p, ctypes.POINTER(RunReport)]
         _LIB = lib
     return _LIB
 
@@ -241,6 +279,52 @@ def _stream_handle(stream) -> int | None:
     if h is not None:
         return h
     raise TypeError("stream must be None, an int handle or a torch.cuda.Stream")
+
+
+def initial_condition(p: Params, which: int = IC_WHICH_ENERGY, profile_dir: str = "initial_profiles") -> np.ndarray:
+    """fsm_initial_condition: the reference's IC dispatch for p.IC_ENERGY
+    (which = IC_WHICH_ENERGY, fm^-4) or p.IC_BARYON (IC_WHICH_BARYON, fm^-3)."""
+    n = p.DIM_X * p.DIM_Y * p.DIM_ETA
+    out = np.zeros(n, dtype=np.float32)
+    _check(_lib().fsm_initial_condition(ctypes.byref(p), which, profile_dir.encode(), _fptr(out), n),
+           "fsm_initial_condition")
+    return out
+
+
+def ic_seed(seed: int = 1):
+    """Restarts the rand() sequence of the fluctuating generators (srand)."""
+    _lib().fsm_ic_seed(seed)
+
+
+def ic_rand() -> int:
+    return int(_lib().fsm_ic_rand())
+
+
+def read_profile(p: Params, fmt: int, path: str) -> np.ndarray:
+    """fsm_read_profile: one of the reference's profile file formats."""
+    n = p.DIM_X * p.DIM_Y * p.DIM_ETA
+    out = np.zeros(n, dtype=np.float32)
+    _check(_lib().fsm_read_profile(ctypes.byref(p), fmt, path.encode(), _fptr(out), n), "fsm_read_profile")
+    return out
+
+
+def write_scalar(p: Params, path: str, var: np.ndarray):
+    var = np.ascontiguousarray(var, dtype=np.float32)
+    _check(_lib().fsm_write_scalar(ctypes.byref(p), path.encode(), _fptr(var)), "fsm_write_scalar")
+
+
+def write_projection(p: Params, path: str, var: np.ndarray):
+    var = np.ascontiguousarray(var, dtype=np.float32)
+    _check(_lib().fsm_write_projection(ctypes.byref(p), path.encode(), _fptr(var)), "fsm_write_projection")
+
+
+def energy_sums(p: Params, T00: np.ndarray, e: np.ndarray):
+    """(total energy after streaming, energy inside the freeze-out surface)."""
+    T00 = np.ascontiguousarray(T00, dtype=np.float32)
+    e = np.ascontiguousarray(e, dtype=np.float32)
+    tot, ins = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    _check(_lib().fsm_energy_sums(ctypes.byref(p), _fptr(T00), _fptr(e), _fptr(tot), _fptr(ins)), "fsm_energy_sums")
+    return float(tot[0]), float(ins[0])
 
 
 class Context:
@@ -312,6 +396,26 @@ class Context:
                       stream=None):
         _check(_lib().fsm_landau_device(self._ctx, ctypes.byref(p), d_T, d_e, d_u, d_p, d_Pi, d_pi,
                                         _stream_handle(stream)), "fsm_landau_device")
+
+    def diagnostics_device(self, p: Params, d_e: int, d_u: int, d_p: int, d_Pi: int, d_pi: int, out: DiagOut,
+                           stream=None):
+        _check(_lib().fsm_diagnostics_device(self._ctx, ctypes.byref(p), d_e, d_u, d_p, d_Pi, d_pi,
+                                             ctypes.byref(out), _stream_handle(stream)), "fsm_diagnostics_device")
+
+    def run_files(self, p: Params, e: np.ndarray, nB: np.ndarray = None, output_dir: str | None = None):
+        """fsm_run_files: the whole reference run (outputs, report, and the
+        output/*.dat files when output_dir exists).  Returns (outputs dict,
+        finalized params, report dict)."""
+        e = np.ascontiguousarray(e, dtype=np.float32).ravel()
+        nb = None if nB is None else np.ascontiguousarray(nB, dtype=np.float32).ravel()
+        q = p.copy()
+        outs = [np.empty(e.size, dtype=np.float64) for _ in range(NUM_OUTPUTS)]
+        arr = (ctypes.c_void_p * NUM_OUTPUTS)(*[o.ctypes.data for o in outs])
+        rep = RunReport()
+        _check(_lib().fsm_run_files(self._ctx, ctypes.byref(q), _fptr(e), None if nb is None else _fptr(nb), e.size,
+                                    arr, None if output_dir is None else output_dir.encode(), ctypes.byref(rep)),
+               "fsm_run_files")
+        return dict(zip(OUTPUT_NAMES, outs)), q, rep.as_dict()
 
     def sync(self, stream=None):
         _check(_lib().fsm_sync(self._ctx, _stream_handle(stream)), "fsm_sync")
@@ -506,18 +610,29 @@ class FreestreamMilne:
     """Mirror of class FREESTREAMMILNE (src/FreestreamMilne.cpp:28-87).
 
     ``run_freestream_milne`` reads ``./freestream_input`` (or ``input_file``)
-    positionally over the built-in defaults, exactly as the reference does, then
-    runs the step on the GPU for IC_ENERGY = 5 (vector input).  A vector whose
-    size differs from DIM raises ``FsmError(FSM_ERR_SIZE)`` where the reference
-    calls exit(-1) (src/FreestreamMilne.cpp:259-264)."""
+    positionally over the built-in defaults, exactly as the reference does,
+    dispatches the initial condition on IC_ENERGY / IC_BARYON
+    (src/FreestreamMilne.cpp:231-298: 5 = the vector of
+    ``initialize_from_vector``, the others from the reference's generators and
+    ``initial_profiles/`` files), runs the step on the GPU and writes the
+    reference's ``output/*.dat`` files when ``output/`` exists.  A vector
+    whose size differs from DIM raises ``FsmError(FSM_ERR_SIZE)`` where the
+    reference calls exit(-1) (:259-264); an unknown initial condition raises
+    ``FsmError(FSM_ERR_IC)`` where the reference prints "Not a valid initial
+    Condition" and returns 0."""
 
-    def __init__(self, device: int = 0, input_file: str = "freestream_input", context: Context | None = None):
+    def __init__(self, device: int = 0, input_file: str = "freestream_input", context: Context | None = None,
+                 output_dir: str | None = "output", profile_dir: str = "initial_profiles"):
         self._ctx = context if context is not None else Context(device)
         self.input_file = input_file
+        self.output_dir = output_dir
+        self.profile_dir = profile_dir
         self.gridSize = 0
         self.tau_LandauMatch = 0.0
         self.init_energy_density = np.zeros(0, dtype=np.float32)
+        self.init_baryon_density = None
         self.params = None
+        self.report = None
         for name in _FINAL_NAMES:
             setattr(self, name, np.zeros(0, dtype=np.float64))
 
@@ -527,11 +642,22 @@ class FreestreamMilne:
     def run_freestream_milne(self, params: Params | None = None) -> int:
         if params is None:
             params, _ = params_read(self.input_file)
-        if params.IC_ENERGY != 5:
-            raise FsmError(FSM_ERR_IC, "run_freestream_milne (only IC_ENERGY=5 vector input)")
-        outs, q, tau = self._ctx.run(params, self.init_energy_density)
+        params = params.copy()
+        params.DIM = params.DIM_X * params.DIM_Y * params.DIM_ETA
+        if params.IC_ENERGY == 5:
+            e = self.init_energy_density
+            if e.size != params.DIM:
+                raise FsmError(FSM_ERR_SIZE, "run_freestream_milne: vector size %d, DIM %d" % (e.size, params.DIM))
+        else:
+            e = initial_condition(params, IC_WHICH_ENERGY, self.profile_dir)
+        nb = None
+        if params.BARYON:
+            nb = (self.init_baryon_density if self.init_baryon_density is not None
+                  else initial_condition(params, IC_WHICH_BARYON, self.profile_dir))
+        outs, q, rep = self._ctx.run_files(params, e, nb, self.output_dir)
         self.params = q
-        self.tau_LandauMatch = tau
+        self.report = rep
+        self.tau_LandauMatch = rep["tau_landau_match"]
         for name, key in zip(_FINAL_NAMES, OUTPUT_NAMES):
             setattr(self, name, outs[key])
         return 0
@@ -560,5 +686,7 @@ __all__ = [
     "Params", "Context", "FreestreamMilne", "FsmError", "OUTPUT_NAMES", "NUM_OUTPUTS", "NUM_OUTPUTS_BARYON",
     "params_default",
     "params_read", "params_finalize", "lib", "version", "LIB_PATH", "PATH_FP32", "PATH_MIXED", "TILES_AUTO",
-    "TILES_BATCH",
+    "TILES_BATCH", "RunReport", "DiagOut", "initial_condition", "ic_seed", "ic_rand", "read_profile", "write_scalar",
+    "write_projection", "energy_sums", "IC_WHICH_ENERGY", "IC_WHICH_BARYON", "PROFILE_XYEV", "PROFILE_CPUVH",
+    "PROFILE_SUPERMC_BLOCK", "PROFILE_TRENTO_BLOCK", "PROFILE_TRENTO3D_BLOCK",
 ]

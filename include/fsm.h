@@ -174,7 +174,10 @@ int fsm_autotune(fsm_ctx *ctx, const fsm_params *p, int *variant);
  * output_to_vectors() (src/FreestreamMilne.cpp:102-136).
  *   p        in: parameters as read (defaults + file); out: finalized
  *            (DIM, TAU, DTAU and DIM_PHIP overrides applied).
- *   e_GeV    initial energy density, n floats, GeV/fm^3.
+ *   e_GeV    initial energy density, n floats, GeV/fm^3 (IC_ENERGY = 5,
+ *            the reference's vector input); for any other IC_ENERGY the
+ *            fm^-4 profile made by fsm_initial_condition / fsm_read_profile,
+ *            used as is.
  *   out      FSM_NUM_OUTPUTS caller-owned arrays of DIM doubles each, in
  *            the order documented above.  When OUTPUTFORMAT is not 2 or 3
  *            they are zero-filled, as the reference leaves them
@@ -319,6 +322,103 @@ int fsm_server_cancel(fsm_server *s, int64_t id);
 /* Counters: requests admitted, completed, launches, bytes of HBM in use. */
 int fsm_server_stats(fsm_server *s, int64_t *admitted, int64_t *completed, int64_t *launches,
                      size_t *bytes_in_use);
+
+/* ------------------------------------------------------------------ */
+/* Input adapters (SURVEY next-row f2): the reference's initial conditions and
+ * profile readers, as host functions that fill a DIM-float vector in cell
+ * order.  A vector made here is the input of fsm_run / fsm_run_device with
+ * the same params: for IC_ENERGY != 5 the library takes the input as the
+ * fm^-4 density itself (no GeV conversion and no 1e-7 floor, as the
+ * reference's dispatch, src/FreestreamMilne.cpp:231-281). */
+enum { FSM_IC_WHICH_ENERGY = 0, FSM_IC_WHICH_BARYON = 1 };
+/* The initial-condition dispatch, src/FreestreamMilne.cpp:231-298.
+ * Energy (p->IC_ENERGY): 1 smooth elliptical Gaussian
+ * (initializeEllipticalGauss(15, 15, 15), src/InitialConditions.cpp:47-72),
+ * 2 its fluctuating form (initializeEllipticalMCGauss, :100-125), 3 the
+ * "x y eta value" file <profile_dir>/e.dat (readDensityFile,
+ * src/FileIO.cpp:133-164), 4 the CPU-VH file <profile_dir>/e.dat
+ * (readEnergyDensityCPUVH, :314-355), 6 two fluctuating Gaussians
+ * (initialize2Gaussians, :238-272; its norm is 0).  Baryon (p->IC_BARYON):
+ * 1 initializeEllipticalGauss(2, 3, 1), 2 its fluctuating form, 3
+ * <profile_dir>/nB.dat.  IC_ENERGY = 5 (the caller's vector) and unknown
+ * codes return FSM_ERR_IC; a missing file FSM_ERR_FILE.  Units fm^-4
+ * (energy) and fm^-3 (baryon). */
+int fsm_initial_condition(const fsm_params *p, int which, const char *profile_dir, float *out, size_t n);
+/* The fluctuating generators draw from the C library's rand() sequence of
+ * the reference build (glibc, unseeded, i.e. seed 1), restated here so that a
+ * fresh process reproduces the reference's profiles.  fsm_ic_seed restarts
+ * it (srand semantics); fsm_ic_rand draws one value (0 .. 2^31-1). */
+void fsm_ic_seed(unsigned seed);
+int fsm_ic_rand(void);
+/* Profile readers (src/FileIO.cpp:133-164, src/InitialConditions.cpp:127-355). */
+enum {
+  FSM_PROFILE_XYEV = 0,           /* readDensityFile: "x y eta value" lines */
+  FSM_PROFILE_CPUVH = 1,          /* readEnergyDensityCPUVH: x y eta value in eta, y, x order, +1e-7 */
+  FSM_PROFILE_SUPERMC_BLOCK = 2,  /* readEnergyDensitySuperMCBlock: x rows of y values, eta envelope */
+  FSM_PROFILE_TRENTO_BLOCK = 3,   /* readEnergyDensityTRENTOBlock: 12 header lines, y rows of x values */
+  FSM_PROFILE_TRENTO3D_BLOCK = 4  /* readEnergyDensityTRENTO3DBlock: x, y, eta order, +1e-3 */
+};
+int fsm_read_profile(const fsm_params *p, int format, const char *path, float *out, size_t n);
+
+/* ------------------------------------------------------------------ */
+/* OUTPUTFORMAT=1 text writers (SURVEY next-row f3), byte-for-byte the
+ * reference's: writeScalarToFile / writeVectorToFile (src/FileIO.cpp:11-81,
+ * "x y eta value" per cell) and the eta = 0 projections
+ * (writeScalarToFileProjection, :85-131).  `path` is the file to write (the
+ * reference writes output/<name>.dat under the working directory). */
+int fsm_write_scalar(const fsm_params *p, const char *path, const float *var);
+int fsm_write_projection(const fsm_params *p, const char *path, const float *var);
+/* Energy after streaming and inside the freeze-out surface
+ * (src/FreestreamMilne.cpp:531-553), as the reference's serial float sums of
+ * T^tautau (host arrays, p finalized); fsm_write_energy_inside writes
+ * output/energy_inside_switch_surf.dat (:555-559). */
+int fsm_energy_sums(const fsm_params *p, const float *T00, const float *e, float *total, float *inside);
+int fsm_write_energy_inside(const char *path, float inside);
+
+/* ------------------------------------------------------------------ */
+/* Diagnostics (SURVEY next-row f4) on the device, from the float hydro
+ * fields of fsm_landau_device (all device pointers, DIM floats per plane;
+ * u has 4 planes, pi 10).  Outputs (any may be NULL):
+ *   R_Pi_Inv, R_pimunu_Inv   calculateBulkInvReynolds / calculateShearInvReynolds
+ *                            (src/HydroValidity.cpp:8-39)
+ *   pi_dot_u[4], pi_mu_mu    calculate_pi_dot_u / calculate_pi_mu_mu (:41-101)
+ *   beta[4], omega[6]        calculateThermalVorticityTensor
+ *                            (src/LandauMatch.cpp:395-489, with the conformal
+ *                            T(e) of src/EquationOfState.cpp:83-94); like the
+ *                            reference it indexes cells as DIM_Y*ix + iy over
+ *                            the interior of the transverse grid and leaves the
+ *                            other entries 0. */
+typedef struct fsm_diag_out {
+  float *R_Pi_Inv;
+  float *R_pimunu_Inv;
+  float *pi_dot_u[4];
+  float *pi_mu_mu;
+  float *beta[4];
+  float *omega[6];
+} fsm_diag_out;
+int fsm_diagnostics_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e, const float *d_u, const float *d_p,
+                           const float *d_Pi, const float *d_pi, const fsm_diag_out *out, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* The whole reference run with its files: run_freestream_milne()
+ * (src/FreestreamMilne.cpp:139-741) for one event from its initial density
+ * (e_in as fsm_run takes it; nB_in for BARYON = 1, fm^-3): the step, the
+ * diagnostics, the energy sums, the 17 result vectors (out, may be NULL; zero
+ * unless OUTPUTFORMAT is 2 or 3) and, when output_dir names an existing
+ * directory, every output/<name>.dat file the reference writes in the same
+ * format.  rep (may be NULL) receives the values the reference prints. */
+typedef struct fsm_run_report {
+  float total_energy_before;  /* src/FreestreamMilne.cpp:350-354 */
+  float total_energy_after;   /* :531-535 */
+  float energy_inside;        /* :537-545 (fraction = energy_inside / total_energy_after) */
+  float R_Pi_Inv_center;      /* :566-568 */
+  float R_pimunu_Inv_center;
+  float tau_landau_match;     /* :326 */
+  float dtau;                 /* DTAU after the E_DEP_FS rule */
+  int dim_phip;               /* DIM_PHIP after the arc-length rule */
+} fsm_run_report;
+int fsm_run_files(fsm_ctx *ctx, fsm_params *p, const float *e_in, const float *nB_in, size_t n,
+                  double *const out[FSM_NUM_OUTPUTS], const char *output_dir, fsm_run_report *rep);
 
 /* Waits for all work the context queued on `stream`. */
 int fsm_sync(fsm_ctx *ctx, void *stream);

@@ -473,7 +473,10 @@ struct Pipeline {
     const R lower_tolerance = (R)1.0e-7;
     const R rescale = (R)1.0;
     e0.resize(P.DIM);
-    for (int i = 0; i < P.DIM; ++i) e0[i] = (R)v[i] * rescale / (R)kHbarc + lower_tolerance;
+    // the vector input (IC_ENERGY = 5) in GeV/fm^3; the other initial
+    // conditions hand their fm^-4 profile over as is (:231-256)
+    for (int i = 0; i < P.DIM; ++i)
+      e0[i] = P.IC_ENERGY == 5 ? (R)v[i] * rescale / (R)kHbarc + lower_tolerance : (R)v[i];
   }
 
   // src/FreeStream.cpp:289-350 (density[DIM][DIM_RAP], flat)
@@ -881,7 +884,7 @@ int oracle_params_finalize(fsm_params *p, const float *e_in, long n) {
     // getEnergyDependentTau on the converted profile, serial float sums.
     float numerator = 0.0, denominator = 0.0;
     for (int is = 0; is < p->DIM; ++is) {
-      float eps = e_in[is] * (float)1.0 / (float)kHbarc + (float)1.0e-7;
+      float eps = p->IC_ENERGY == 5 ? e_in[is] * (float)1.0 / (float)kHbarc + (float)1.0e-7 : e_in[is];
       numerator += (eps * eps);
       denominator += eps;
     }
@@ -1000,6 +1003,99 @@ void oracle_landau_condition(long n, const double *T, float TAU, int ntrial, dou
     kappa_e[i] = ke;
     kappa_u[i] = ku;
   }
+}
+
+// Diagnostics of the reference on its float hydro fields (D cells; u 4 planes,
+// pi 10 planes): out = 17 planes of D floats in the order R_Pi_Inv,
+// R_pimunu_Inv, pi_dot_u[4], pi_mu_mu, beta[4], omega[6] (zero where the
+// reference leaves its calloc'd arrays untouched).
+//   calculateBulkInvReynolds    src/HydroValidity.cpp:8-20
+//   calculateShearInvReynolds   src/HydroValidity.cpp:22-39
+//   calculate_pi_dot_u          src/HydroValidity.cpp:41-77
+//   calculate_pi_mu_mu          src/HydroValidity.cpp:79-101
+//   calculateThermalVorticityTensor src/LandauMatch.cpp:395-477 (2-D index
+//                               DIM_Y*ix + iy over the transverse interior)
+//   temperatureFromEnergyDensity src/EquationOfState.cpp:83-94
+static float oracle_temperature(float eps) {
+  float a = 13.8997;
+  float T = powf((eps / a), 0.25);
+  return T;
+}
+
+void oracle_diagnostics(const fsm_params *p, const float *e, const float *u, const float *pr, const float *Pi,
+                        const float *pi, float *out) {
+  const long D = (long)p->DIM_X * p->DIM_Y * p->DIM_ETA;
+  const float TAU = p->TAU;
+  for (long k = 0; k < 17 * D; ++k) out[k] = 0.0f;
+  float *R_Pi = out, *R_pi = out + D, *pdu = out + 2 * D, *pmm = out + 6 * D, *beta = out + 7 * D,
+        *omega = out + 11 * D;
+  for (long is = 0; is < D; ++is) {
+    float pp;
+    if (pr[is] < 1.0e-5) pp = 1.0e-5;
+    else pp = pr[is];
+    R_Pi[is] = Pi[is] / pp;
+  }
+  const float tau2 = TAU * TAU;
+  for (long is = 0; is < D; ++is) {
+    const float *s0 = pi;
+    float num = s0[is] * s0[is] -
+                2.0 * (s0[D + is] * s0[D + is] + s0[2 * D + is] * s0[2 * D + is] + tau2 * s0[3 * D + is] * s0[3 * D + is]) +
+                (s0[4 * D + is] * s0[4 * D + is] + s0[7 * D + is] * s0[7 * D + is] +
+                 tau2 * tau2 * s0[9 * D + is] * s0[9 * D + is]) +
+                2.0 * (s0[5 * D + is] * s0[5 * D + is] + tau2 * s0[6 * D + is] * s0[6 * D + is] +
+                       tau2 * s0[8 * D + is] * s0[8 * D + is]);
+    float den;
+    if (e[is] < 1.0e-5) den = 1.0e-5;
+    else den = e[is] * e[is] + 3.0 * pr[is] * pr[is];
+    R_pi[is] = sqrt(fabs(num)) / sqrt(fabs(den));
+  }
+  for (long is = 0; is < D; ++is) {
+    float ut = u[is], ux = u[D + is], uy = u[2 * D + is], un = u[3 * D + is];
+    ux = -ux;
+    uy = -uy;
+    un = -TAU * TAU * un;
+    float pitt = pi[is], pitx = pi[D + is], pity = pi[2 * D + is], pitn = pi[3 * D + is], pixx = pi[4 * D + is],
+          pixy = pi[5 * D + is], pixn = pi[6 * D + is], piyy = pi[7 * D + is], piyn = pi[8 * D + is],
+          pinn = pi[9 * D + is];
+    pdu[is] = pitt * ut + pitx * ux + pity * uy + pitn * un;
+    pdu[D + is] = pitx * ut + pixx * ux + pixy * uy + pixn * un;
+    pdu[2 * D + is] = pity * ut + pixy * ux + piyy * uy + piyn * un;
+    pdu[3 * D + is] = pitn * ut + pixn * ux + piyn * uy + pinn * un;
+    pixx = -pixx;
+    piyy = -piyy;
+    pinn = -TAU * TAU * pinn;
+    pmm[is] = pitt + pixx + piyy + pinn;
+  }
+  const int NX = p->DIM_X, NY = p->DIM_Y;
+  const float dx = p->DX, dy = p->DY;
+  for (int ix = 1; ix < NX - 1; ++ix)
+    for (int iy = 1; iy < NY - 1; ++iy) {
+      const long is = (long)NY * ix + iy, px = (long)NY * (ix + 1) + iy, mx = (long)NY * (ix - 1) + iy,
+                 py = (long)NY * ix + (iy + 1), my = (long)NY * ix + (iy - 1);
+      float T = oracle_temperature(e[is]);
+      beta[is] = u[is] / T;
+      beta[D + is] = -u[D + is] / T;
+      beta[2 * D + is] = -u[2 * D + is] / T;
+      beta[3 * D + is] = 0.0f / T;
+      float bxpy = -u[D + py] / oracle_temperature(e[py]), bxmy = -u[D + my] / oracle_temperature(e[my]);
+      float bypx = -u[2 * D + px] / oracle_temperature(e[px]), bymx = -u[2 * D + mx] / oracle_temperature(e[mx]);
+      omega[3 * D + is] = 0.5 * ((bxpy - bxmy) / 2.0 / dy - (bypx - bymx) / 2.0 / dx);
+    }
+}
+
+// Serial float sums of src/FreestreamMilne.cpp:531-545 (T00, e in fm^-4).
+void oracle_energy_sums(const fsm_params *p, const float *T00, const float *e, float *tot, float *inside) {
+  const long D = (long)p->DIM_X * p->DIM_Y * p->DIM_ETA;
+  float a = 0.0, b = 0.0;
+  for (long is = 0; is < D; ++is) a += T00[is];
+  if (p->DIM_ETA > 1) a *= (p->TAU * p->DX * p->DY * p->DETA);
+  else a *= (p->TAU * p->DX * p->DY);
+  for (long is = 0; is < D; ++is)
+    if ((e[is] * kHbarc) > p->E_FREEZE) b += T00[is];
+  if (p->DIM_ETA > 1) b *= (p->TAU * p->DX * p->DY * p->DETA);
+  else b *= (p->TAU * p->DX * p->DY);
+  *tot = a;
+  *inside = b;
 }
 
 int oracle_num_threads(void) {

@@ -170,3 +170,40 @@ def landau_condition(T, tau: float, ntrial: int = 4, rel: float = 1e-6, seed: in
     ke, ku = np.empty(n), np.empty(n)
     lib.oracle_landau_condition(n, _d(T), tau, ntrial, rel, seed, 1 if normwise else 0, _d(ke), _d(ku))
     return ke, ku
+
+
+def _diag_lib():
+    lib = _lib()
+    if not hasattr(lib, "_diag_ready"):
+        from fsm_amd import Params
+
+        fp = ctypes.POINTER(ctypes.c_float)
+        lib.oracle_diagnostics.argtypes = [ctypes.POINTER(Params), fp, fp, fp, fp, fp, fp]
+        lib.oracle_diagnostics.restype = None
+        lib.oracle_energy_sums.argtypes = [ctypes.POINTER(Params), fp, fp, fp, fp]
+        lib.oracle_energy_sums.restype = None
+        lib._diag_ready = True
+    return lib
+
+
+def diagnostics(p, e, u, pr, Pi, pi):
+    """The reference's diagnostics on float hydro fields (see
+    fsm_oracle.cpp): dict of R_Pi_Inv, R_pimunu_Inv, pi_dot_u (4, D),
+    pi_mu_mu, beta (4, D), omega (6, D)."""
+    D = p.DIM_X * p.DIM_Y * p.DIM_ETA
+    f = [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in (e, u, pr, Pi, pi)]
+    out = np.zeros(17 * D, dtype=np.float32)
+    _diag_lib().oracle_diagnostics(ctypes.byref(p), *[_f(a) for a in f], _f(out))
+    out = out.reshape(17, D)
+    return {"R_Pi_Inv": out[0], "R_pimunu_Inv": out[1], "pi_dot_u": out[2:6], "pi_mu_mu": out[6],
+            "beta": out[7:11], "omega": out[11:17]}
+
+
+def energy_sums(p, T00, e):
+    """Serial float sums of the energy after streaming and inside the
+    freeze-out surface (T00, e in fm^-4)."""
+    T00 = np.ascontiguousarray(T00, dtype=np.float32)
+    e = np.ascontiguousarray(e, dtype=np.float32)
+    a, b = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    _diag_lib().oracle_energy_sums(ctypes.byref(p), _f(T00), _f(e), _f(a), _f(b))
+    return float(a[0]), float(b[0])
