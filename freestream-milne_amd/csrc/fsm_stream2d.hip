@@ -532,18 +532,18 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
   const long P = ev.t_plane;
   const float inv_sc = 1.0f / ev.mscale[FSM_KS_PARTS];  // a power of two: exact
   constexpr int CPT = (NC + 255) / 256;     // cells per thread
+  // cell q of this thread: c = tid + 256 q, column c / TY, row c % TY (TY =
+  // 32: shifts, recomputed where used rather than held in registers)
+  auto cell_x = [&](int q) { return x0 + (tid + 256 * q) / TY; };
+  auto cell_y = [&](int q) { return y0 + (tid + 256 * q) % TY; };
+  auto cell_live = [&](int q) { return tid + 256 * q < NC && cell_x(q) < nx && cell_y(q) < ny; };
   float t[CPT][6];
-  int cix[CPT], ciy[CPT];
-  bool live[CPT];
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
     const int c = tid + 256 * q;
-    const int lx = c / TY, ly = c - lx * TY;
-    cix[q] = x0 + lx;
-    ciy[q] = y0 + ly;
-    live[q] = c < NC && cix[q] < nx && ciy[q] < ny;
+    const bool lv = cell_live(q);
 #pragma unroll
-    for (int m = 0; m < 6; ++m) t[q][m] = live[q] ? Tt[m * NC + c] * inv_sc : 0.0f;
+    for (int m = 0; m < 6; ++m) t[q][m] = lv ? Tt[m * NC + c] * inv_sc : 0.0f;
   }
   // rounding-dependent samples next to the mask boundary: FP32, the
   // reference's masks (the cells of a thread are evaluated together)
@@ -551,7 +551,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     const Stencil2D S = ev.st[ev.amb[a]];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-      const float v = live[q] ? sample_correction(ev, S, cix[q], ciy[q]) : 0.0f;
+      const float v = cell_live(q) ? sample_correction(ev, S, cell_x(q), cell_y(q)) : 0.0f;
       t[q][0] += v;
       t[q][1] = fmaf(v, S.c, t[q][1]);
       t[q][2] = fmaf(v, S.s, t[q][2]);
@@ -563,10 +563,10 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
   // src/LandauMatch.cpp:104: T = sum * d_phip / TAU; rows 3, 6, 8, 9 carry tanh(0) = 0
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
-    if (!live[q]) continue;
+    if (!cell_live(q)) continue;
     if (ev.out_j) {
       // src/LandauMatch.cpp:140: J = sum * DRAP * d_phip; rows 1, c, s, tanh(0) = 0
-      const long is = (long)cix[q] * ny + ciy[q];
+      const long is = (long)cell_x(q) * ny + cell_y(q);
 #pragma unroll
       for (int c = 0; c < 3; ++c) ev.T[c * P + is] = ((t[q][c] * ev.out_scale) * ev.jscale) * ev.d_phip;
       ev.T[3 * P + is] = 0.0f;
@@ -575,7 +575,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     float rr[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) rr[m] = ((t[q][m] * ev.out_scale) * ev.d_phip) / ev.tau;
-    const long is = (long)cix[q] * ny + ciy[q];
+    const long is = (long)cell_x(q) * ny + cell_y(q);
     ev.T[0 * P + is] = rr[0];
     ev.T[1 * P + is] = rr[1];
     ev.T[2 * P + is] = rr[2];
