@@ -509,8 +509,15 @@ __device__ __forceinline__ double jrsq(double x) {
   return fma(0.5 * y, r, y);
 }
 
+// `w` (optional): a velocity near the Landau frame's (spatial components in
+// the orthonormal frame), from which the sweeps start: A is first boosted by
+// B(-w), the pure boost to that velocity's rest frame, built from w with
+// gamma recomputed so that it is a Lorentz transformation to working
+// precision, and M starts at B(w).  The sweeps then only remove what is left
+// of A^{0i}; the frame they converge to is the same.
 template <typename R>
-__device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4]) {
+__device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4],
+                             const R *w = nullptr) {
   const R tau = (R)TAU;
   R A[4][4], M[4][4];
   A[0][0] = t[0];
@@ -527,6 +534,36 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) M[i][j] = (i == j) ? (R)1 : (R)0;
+  if (w) {
+    const R v[4] = {(R)0, w[0], w[1], w[2]};
+    const R g = sqrt((R)1 + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+    const R h = (R)1 / ((R)1 + g);
+    // B(w) = [[g, w^T], [w, I + w w^T / (1 + g)]]; B(-w) flips the 0i signs
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        M[i][j] = (i == 0 && j == 0) ? g : (i == 0) ? v[j] : (j == 0) ? v[i] : ((i == j ? (R)1 : (R)0) + h * v[i] * v[j]);
+    R C[4][4];  // A B(-w)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        R acc = (R)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc += A[i][k] * (((k == 0) != (j == 0)) ? -M[k][j] : M[k][j]);
+        C[i][j] = acc;
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = i; j < 4; ++j) {
+        R acc = (R)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc += (((i == 0) != (k == 0)) ? -M[i][k] : M[i][k]) * C[k][j];
+        A[i][j] = A[j][i] = acc;
+      }
+  }
   bool ok = true, conv = false;
   for (int sweep = 0; sweep < 60 && ok; ++sweep) {
     const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
@@ -591,8 +628,12 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
   return 0;
 }
 
-__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4]) {
-  (void)landau_jacobi<double>(t, TAU, 1.0e-28, 1.0e-12, e_out, u_out);
+// FP64 solve: converged when the off-diagonal row is below 1e-12 of A^{00}
+// (on sampled cells of a lumpy 2-D and a 3-D event its float results equal
+// those of a 1e-14 criterion; the eigenvalue error is quadratic in the
+// residual).
+__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4], const double *w = nullptr) {
+  (void)landau_jacobi<double>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out, w);
 }
 
 // FP32 stage of the mixed path.  Returns true when FP32 decides the result to
@@ -601,8 +642,9 @@ __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_
 // small difference of large terms (kappa > 16: nearly null tensors in dilute
 // edge cells, where FP32 rounding in T is amplified by kappa).  Those cells
 // are solved again in FP64.
-__device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float u_out[4]) {
+__device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float u_out[4], int *st_out) {
   const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
+  *st_out = st;
   // First-order sensitivity of e = T_{mu nu} u^mu u^nu to a relative change
   // of every component: kappa = sum |u_mu| |T^{mu nu}| |u_nu| / e (>= u0^2 T^tt / e).
   const float a0 = fabsf(u_out[0]), a1 = fabsf(u_out[1]), a2 = fabsf(u_out[2]), a3 = TAU * TAU * fabsf(u_out[3]);
@@ -661,7 +703,8 @@ struct LandauShared {
   int n64;
   int who[256];   // threads whose cell needs FP64
   int tidx[256];  // T index of each listed thread's cell
-  float e[256], u[4][256];
+  int warm[256];  // 1: the FP32 frame converged and seeds the FP64 sweeps
+  float e[256], u[4][256];  // FP32 frame in, FP64 result out
 };
 
 // Landau frame of the workgroup's cells (one per thread, `live` marks real
@@ -669,7 +712,8 @@ struct LandauShared {
 // every cell.  Mixed path: FP32 Jacobi for every cell, then the cells that
 // FP32 cannot decide are gathered into a dense list and re-solved in FP64 by
 // the first threads of the workgroup, so FP64 work is not spread over
-// partly-idle waves.
+// partly-idle waves; a converged FP32 frame is the FP64 sweeps' starting
+// point (about half the sweeps of a cold start).
 __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, const float t[10], float *e,
                                    float u[4], LandauShared &S) {
   if (!ev.mixed) {
@@ -677,7 +721,8 @@ __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, 
     return;
   }
   bool need = false;
-  if (live) need = !landau_fp32_ok(t, ev.tau, e, u);
+  int st = 0;
+  if (live) need = !landau_fp32_ok(t, ev.tau, e, u, &st);
   __syncthreads();  // the previous group's list is no longer read
   if (threadIdx.x == 0) S.n64 = 0;
   __syncthreads();
@@ -685,6 +730,9 @@ __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, 
     const int k = atomicAdd(&S.n64, 1);
     S.who[k] = threadIdx.x;
     S.tidx[threadIdx.x] = (int)tidx;
+    S.warm[threadIdx.x] = st;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) S.u[c][threadIdx.x] = u[c];
   }
   __syncthreads();
   const int n = S.n64;
@@ -694,7 +742,9 @@ __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, 
     float tt[10], e64, u64[4];
 #pragma unroll
     for (int c = 0; c < 10; ++c) tt[c] = ev.T[c * ev.t_plane + ti];
-    landau_fp64(tt, ev.tau, &e64, u64);
+    // the FP32 frame (orthonormal components) seeds the FP64 sweeps
+    const double w3[3] = {(double)S.u[1][w], (double)S.u[2][w], (double)ev.tau * (double)S.u[3][w]};
+    landau_fp64(tt, ev.tau, &e64, u64, S.warm[w] ? w3 : nullptr);
     S.e[w] = e64;
 #pragma unroll
     for (int c = 0; c < 4; ++c) S.u[c][w] = u64[c];
@@ -830,7 +880,9 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds,
   landau_cell_compute(ev, live, (long)ie * ev.nxy + xy, r, S);  // starts with a barrier
   if (live) {
     const int li = xl * FSML_PITCH + el;
-    for (int c = 0; c < np; ++c) lds[c * FSML_PLANE + li] = cell_value(r, c);
+#pragma unroll
+    for (int c = 0; c < FSM_NUM_OUTPUTS + 5; ++c)
+      if (c < np) lds[c * FSML_PLANE + li] = cell_value(r, c);
     cell_store_float(ev, xy * ev.neta + ie, r);
   }
   __syncthreads();
