@@ -331,6 +331,46 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ in,
 // they read.  Every read lies in the window: a kept cell's source rows
 // ix+ox, ix+ox+1 lie in [0, NX] and columns iy+oy, iy+oy+1 in [0, NY]
 // (checked on the host).
+// Stages window rows [lo, lo + nr) of Q(x, y) = w0 (E0(x, y) rf0) + w1 (E1(x, y) rf1)
+// (pitch ny + 1; rows outside the grid and column ny are zero).  The rows
+// are walked as one flat range, eight elements per thread per round, with
+// every load issued before the first use (out-of-range elements load element
+// 0 and store 0), so a restage costs about one memory latency per round
+// rather than one per row.
+__device__ __forceinline__ void stage_q_rows(float *q, const float *__restrict__ p0, const float *__restrict__ p1,
+                                             float w0, float w1, float rf0, float rf1, int lo, int nr, int nx,
+                                             int ny) {
+  constexpr int U = 8;
+  const int pitch = ny + 1, n = nr * pitch;
+  // element k of the window is row k / pitch, column k % pitch, walked incrementally
+  const int dr = FSM3D_THREADS / pitch, dc = FSM3D_THREADS - dr * pitch;
+  int r = (int)threadIdx.x / pitch, c = (int)threadIdx.x - r * pitch;
+  for (int k0 = (int)threadIdx.x; k0 < n; k0 += U * FSM3D_THREADS) {
+    float e0[U], e1[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int gx = lo + r;
+      in[u] = k0 + u * FSM3D_THREADS < n && (unsigned)gx < (unsigned)nx && c < ny;
+      const int src = in[u] ? gx * ny + c : 0;
+      e0[u] = p0[src];
+      e1[u] = p1[src];
+      r += dr;
+      c += dc;
+      if (c >= pitch) {
+        c -= pitch;
+        ++r;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * FSM3D_THREADS;
+      if (k >= n) break;
+      q[k] = in[u] ? w0 * (e0[u] * rf0) + w1 * (e1[u] * rf1) : 0.0f;
+    }
+  }
+}
+
 __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int nx = ev.nx, ny = ev.ny, nxy = nx * ny;
   const int pitch = ny + 1;
@@ -338,7 +378,6 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int c0 = tile * FSM3D_TILE;
   const int row_lo = c0 / ny;
   const int row_hi = min(c0 + FSM3D_TILE - 1, nxy - 1) / ny;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int cx[FSM3D_M], cy[FSM3D_M], ba[FSM3D_M];
   bool live[FSM3D_M];
 #pragma unroll
@@ -391,17 +430,7 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
         __syncthreads();  // previous window no longer read
         wlo = lo;
         whi = hi;
-        for (int r = wave; r <= hi - lo; r += FSM3D_THREADS / 64) {
-          const int gx = lo + r;
-          const bool rin = gx >= 0 && gx < nx;
-          const float *r0p = p0 + (long)gx * ny, *r1p = p1 + (long)gx * ny;
-          float *qr = q + r * pitch;
-          for (int y = lane; y < pitch; y += 64) {
-            float v = 0.0f;
-            if (rin && y < ny) v = w0 * (r0p[y] * S.rf0) + w1 * (r1p[y] * S.rf1);
-            qr[y] = v;
-          }
-        }
+        stage_q_rows(q, p0, p1, w0, w1, S.rf0, S.rf1, lo, hi - lo + 1, nx, ny);
         __syncthreads();
       }
       // branch-free over the lane's cells: a dropped cell reads element 0 of

@@ -1,6 +1,6 @@
 """Mixed-precision path (FSM_PATH_MIXED): the 2-D quadrature on FP16 MFMA with
 hi+lo split knots and weights, and FP32 Landau sweeps with FP64 for dilute
-and high-gamma cells.  Same bar as the FP32 path: T^{mu nu} within 1e-4 of
+and high-gamma cells (in 3-D the quadrature stays on the FP32 stream kernel).  Same bar as the FP32 path: T^{mu nu} within 1e-4 of
 max T^{tau tau} against the reference float path (oracle) and, in 2-D,
 against the FP64 oracle on cells whose samples all stay inside the grid
 (float and FP64 keep different samples at the mask boundary); e, u, pi, Pi
@@ -42,6 +42,19 @@ def _case(name):
         return p, e.ravel()
     if name == "small3d":
         return W.small_3d()
+    if name == "small3d_int":  # DTAU/DX integer: samples on grid lines
+        return W.small_3d(nx=33, neta=17, nrap=7, nphi=16, dtau=0.5, seed=9)
+    if name == "rect3d":  # DX != DY, odd N_phi, a ring wider than the grid
+        p = W.base_params(DIM_X=57, DIM_Y=43, DIM_ETA=15, DIM_RAP=7, DIM_PHIP=151, DX=0.1, DY=0.08, DETA=0.12,
+                          TAU0=0.1, DTAU=1.9, SIGMA=0.8)
+        e2 = W.hotspots_2d(57, 43, 0.1, 0.08, 15, 31, width=0.4)
+        env = W.eta_envelope(15, 0.12, flat=0.4, tail=0.4)
+        return p, (e2[:, :, None] * env[None, None, :]).astype(np.float32).ravel()
+    if name == "wide3d":  # more rows than one source window holds
+        p = W.base_params(DIM_X=151, DIM_Y=101, DIM_ETA=7, DIM_RAP=5, DIM_PHIP=20, DX=0.1, DY=0.1, DETA=0.1,
+                          TAU0=0.1, DTAU=2.0, SIGMA=1.0)
+        e2 = W.hotspots_2d(151, 101, 0.1, 0.1, 40, 17, width=0.6)
+        return p, (e2[:, :, None] * np.ones((1, 1, 7))).astype(np.float32).ravel()
     if name == "int2d":  # DTAU/DX = 10: samples on grid lines next to the mask edge
         p = W.base_params(DIM_X=101, DIM_Y=101, DIM_PHIP=64, DX=0.08, DY=0.08, TAU0=0.1, DTAU=0.8)
         return p, W.hotspots_2d(101, 101, 0.08, 0.08, 60, 13).ravel()
@@ -62,7 +75,8 @@ def _gpu(ctx, q, e):
     return d_T.cpu().numpy().reshape(10, D), d_out.cpu().numpy().reshape(17, D)
 
 
-@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d"])
+@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d", "small3d_int",
+                                  "rect3d", "wide3d"])
 def test_mixed_parity(mixed_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)
@@ -91,6 +105,21 @@ def test_mixed_parity(mixed_ctx, name):
         assert errs[k] <= TOL, (k, errs[k])
         if q.DIM_ETA == 1:
             assert errs64[k] <= TOL, (k, errs64[k])
+
+
+def test_mixed_parity_c2_full(mixed_ctx):
+    """The 131^2 x 101 3-D event (DIM_RAP 21, N_phi 51) on the mixed path."""
+    p, e = W.config("C2")
+    q = fsm_amd.params_finalize(p, e)
+    print("mixed case C2 DIM", q.DIM, "DIM_PHIP", q.DIM_PHIP, flush=True)
+    T, out = _gpu(mixed_ctx, q, e)
+    ref = O.run(q, e, "float")
+    tf = tmunu_err(T, ref["T"])
+    errs = output_errs(out, ref["out"], np.abs(ref["T"][0]).max(), well_conditioned(ref["T"], q.TAU))
+    print("C2 mixed T err vs float path %.3g" % tf, errs, flush=True)
+    assert tf <= TOL
+    for k in ("e", "p", "u", "pi", "Pi"):
+        assert errs[k] <= TOL, (k, errs[k])
 
 
 def test_mixed_runs_the_mfma_form(mixed_ctx, gpu_ctx):
