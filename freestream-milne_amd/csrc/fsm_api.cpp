@@ -700,7 +700,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     ev.win_rows = std::max(need, FSM3D_WIN_FLOATS / pitch);
     ev.win_rows = std::min(ev.win_rows, nx + 2);
     ev.win_rows = std::max(ev.win_rows, need);
-    ev.lds_floats = ev.win_rows * pitch;
+    ev.lds_floats = (ev.win_rows + 2) * pitch;  // + the 2 x 2 zero patch read by dropped cells
     if ((size_t)ev.lds_floats * sizeof(float) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
   }
   ev.d_phip = (float)((2.0 * M_PI) / (double)(float)nphi);

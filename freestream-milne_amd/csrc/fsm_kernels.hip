@@ -371,6 +371,9 @@ __device__ __forceinline__ void stage_q_rows(float *q, const float *__restrict__
   }
 }
 
+typedef unsigned short fsm_u16x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const float fsm_lds_cf;  // 32-bit LDS addresses
+
 __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int nx = ev.nx, ny = ev.ny, nxy = nx * ny;
   const int pitch = ny + 1;
@@ -378,17 +381,22 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int c0 = tile * FSM3D_TILE;
   const int row_lo = c0 / ny;
   const int row_hi = min(c0 + FSM3D_TILE - 1, nxy - 1) / ny;
-  int cx[FSM3D_M], cy[FSM3D_M], ba[FSM3D_M];
-  bool live[FSM3D_M];
+  // the lane's cells c0 + tid + 256 m as (ix << 16 | iy) (all ones past the
+  // grid: fails every kept-interval test) and as LDS byte address in the
+  // window at zero row offset; a dropped cell reads the 2 x 2 zero patch
+  // after the window (rows W, W+1, columns 0, 1)
+  const unsigned qbase = (unsigned)(size_t)(fsm_lds_cf *)q;
+  unsigned pk[FSM3D_M];
+  unsigned ba[FSM3D_M];
 #pragma unroll
   for (int m = 0; m < FSM3D_M; ++m) {
-    int c = c0 + threadIdx.x + FSM3D_THREADS * m;
-    live[m] = c < nxy;
-    if (!live[m]) c = c0;
-    cx[m] = c / ny;
-    cy[m] = c - cx[m] * ny;
-    ba[m] = cx[m] * pitch + cy[m];
+    const int c = c0 + threadIdx.x + FSM3D_THREADS * m;
+    const int cx = c / ny, cy = c - cx * ny;
+    pk[m] = c < nxy ? ((unsigned)cx << 16) | (unsigned)cy : 0xFFFFFFFFu;
+    ba[m] = qbase + (c < nxy ? 4u * (unsigned)(cx * pitch + cy) : 0u);
   }
+  const unsigned zero_b = qbase + 4u * (unsigned)(W * pitch);
+  const unsigned pitch_b = 4u * (unsigned)pitch;
   float T[FSM3D_M][10];
 #pragma unroll
   for (int m = 0; m < FSM3D_M; ++m)
@@ -431,22 +439,27 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
         wlo = lo;
         whi = hi;
         stage_q_rows(q, p0, p1, w0, w1, S.rf0, S.rf1, lo, hi - lo + 1, nx, ny);
+        if (threadIdx.x < 4) q[W * pitch + (threadIdx.x & 1) + (threadIdx.x >> 1) * pitch] = 0.0f;
         __syncthreads();
       }
-      // branch-free over the lane's cells: a dropped cell reads element 0 of
-      // the window and its value is discarded
-      const int delta = (P.ox - wlo) * pitch + P.oy;
-      const unsigned xs = (unsigned)(P.xhi - P.xlo), ys = (unsigned)(P.yhi - P.ylo);
-      bool ok[FSM3D_M];
+      // branch-free over the lane's cells: a dropped cell reads the zero patch
+      // (ix - xlo <= xhi - xlo and iy - ylo <= yhi - ylo, unsigned per 16-bit half)
+      const unsigned lo = ((unsigned)P.xlo << 16) | (unsigned)P.ylo;
+      const unsigned ext = ((unsigned)(P.xhi - P.xlo) << 16) | (unsigned)(P.yhi - P.ylo);
+      const unsigned delta_b = 4u * (unsigned)((P.ox - wlo) * pitch + P.oy);  // modulo 2^32
       float q00[FSM3D_M], q01[FSM3D_M], q10[FSM3D_M], q11[FSM3D_M];
 #pragma unroll
       for (int m = 0; m < FSM3D_M; ++m) {
-        ok[m] = live[m] & ((unsigned)(cy[m] - P.ylo) <= ys) & ((unsigned)(cx[m] - P.xlo) <= xs);
-        const int ad = ok[m] ? ba[m] + delta : 0;
-        q00[m] = q[ad];
-        q01[m] = q[ad + 1];
-        q10[m] = q[ad + pitch];
-        q11[m] = q[ad + pitch + 1];
+        const fsm_u16x2 t = __builtin_bit_cast(fsm_u16x2, pk[m]) - __builtin_bit_cast(fsm_u16x2, lo);
+        const fsm_u16x2 mn = __builtin_elementwise_min(t, __builtin_bit_cast(fsm_u16x2, ext));
+        const bool ok = __builtin_bit_cast(unsigned, mn) == __builtin_bit_cast(unsigned, t);
+        const unsigned ad = ok ? ba[m] + delta_b : zero_b;
+        fsm_lds_cf *qa = (fsm_lds_cf *)(size_t)ad;
+        fsm_lds_cf *qc = (fsm_lds_cf *)(size_t)(ad + pitch_b);
+        q00[m] = qa[0];
+        q01[m] = qa[1];
+        q10[m] = qc[0];
+        q11[m] = qc[1];
       }
 #pragma unroll
       for (int m = 0; m < FSM3D_M; ++m) {
@@ -454,7 +467,6 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
         v = fmaf(q01[m], P.w01, v);
         v = fmaf(q10[m], P.w10, v);
         v = fmaf(q11[m], P.w11, v);
-        v = ok[m] ? v : 0.0f;
         mo[m][0] += v;
         mo[m][1] = fmaf(v, P.c, mo[m][1]);
         mo[m][2] = fmaf(v, P.s, mo[m][2]);
@@ -482,8 +494,9 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const long P = ev.t_plane;
 #pragma unroll
   for (int m = 0; m < FSM3D_M; ++m) {
-    if (!live[m]) continue;
-    const long idx = (long)ieta * nxy + cx[m] * ny + cy[m];
+    const int c = c0 + threadIdx.x + FSM3D_THREADS * m;
+    if (c >= nxy) continue;
+    const long idx = (long)ieta * nxy + c;
 #pragma unroll
     for (int v = 0; v < 10; ++v) ev.T[v * P + idx] = T[m][v] * ev.d_phip;  // :105
   }
