@@ -928,6 +928,7 @@ int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double 
     if (st) return st;
   }
   EventLandau L = landau_event(*p, T, em, ctx->path == FSM_PATH_MIXED);
+  L.eta_zero = p->DIM_ETA == 1;  // the 2-D stream kernel writes the eta components as zeros
   L.out = d_out;
   if (J) {
     L.J = J;
@@ -1577,7 +1578,7 @@ struct fsm_batch {
   DevBuf<int4> d_items;
   // work-item ranges per stage: knot scale, spline, 2-D stream, 3-D source, 3-D stream, Landau
   int off_ks = 0, n_ks = 0, off_sp = 0, n_sp = 0, off_s2 = 0, n_s2 = 0;
-  int off_src = 0, n_src = 0, off_s3 = 0, n_s3 = 0, off_l = 0, n_l = 0;
+  int off_src = 0, n_src = 0, off_s3 = 0, n_s3 = 0, off_l = 0, n_l = 0, off_l2 = 0, n_l2 = 0;
   size_t lds2 = 0, lds3 = 0, lds_l = 0;  // dynamic LDS of the stream and Landau launches
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1605,7 +1606,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   std::vector<Event2D> e2;
   std::vector<Event3D> e3;
   std::vector<EventLandau> el;
-  std::vector<int4> ks, sp, s2, src, s3, ld;
+  std::vector<int4> ks, sp, s2, src, s3, ld, ld2;
   std::vector<float> s3w;  // work estimate of each 3-D stream item
   size_t bytes = 0;
   for (auto &ev : b->evs)
@@ -1670,6 +1671,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     int em = 0, jm = 0;
     const float *T = add_pass(ev, ev.d_e, false, cur, &em);
     EventLandau L = landau_event(p, T, em, path == FSM_PATH_MIXED);
+    L.eta_zero = p.DIM_ETA == 1;  // the 2-D stream kernel writes the eta components as zeros
     L.out = ev.d_out;
     if (ev.d_nb) {
       L.J = add_pass(ev, ev.d_nb, true, cur, &jm);
@@ -1684,7 +1686,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       b->lds_l = std::max(b->lds_l, fsm_landau_lds_bytes(ev.d_nb != nullptr));
     } else {
       for (int c = 0; c < p.DIM; c += kLandauChunk)
-        ld.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
+        (L.eta_zero ? ld2 : ld).push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
     }
   }
   {
@@ -1709,6 +1711,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   put(src, &b->off_src, &b->n_src);
   put(s3, &b->off_s3, &b->n_s3);
   put(ld, &b->off_l, &b->n_l);
+  put(ld2, &b->off_l2, &b->n_l2);
   if (!b->async_upload) {
     if (!e2.empty()) FSM_TRY(b->d_ev2.upload(e2));
     if (!e3.empty()) FSM_TRY(b->d_ev3.upload(e3));
@@ -1750,6 +1753,7 @@ int batch_launch(fsm_batch *b, hipStream_t s, bool timed) {
   FSM_TRY(fsm_launch_stream3d_batch(b->d_ev3.p, it + b->off_s3, b->n_s3, b->lds3, s));
   if (timed) tmark(ctx, 2, s);
   FSM_TRY(fsm_launch_landau_batch(b->d_evl.p, it + b->off_l, b->n_l, b->lds_l, s));
+  FSM_TRY(fsm_launch_landau2d_batch(b->d_evl.p, it + b->off_l2, b->n_l2, s));
   if (timed) tmark(ctx, 3, s);
   return FSM_OK;
 }

@@ -41,6 +41,8 @@ hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int 
 size_t fsm_landau_lds_bytes(int baryon);
 hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, size_t lds_bytes,
                                    hipStream_t s);
+// 2-D items of events with eta_zero set (the 3x3 Landau frame).
+hipError_t fsm_launch_landau2d_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s);
 // Diagnostics (fsm_diag.hip).
 hipError_t fsm_launch_diagnostics(const fsm_params &p, const float *e, const float *u, const float *pr,
                                   const float *Pi, const float *pi, const fsm_diag_out &out, hipStream_t s);

@@ -557,64 +557,78 @@ __device__ __forceinline__ double jrsq(double x) {
 // gamma recomputed so that it is a Lorentz transformation to working
 // precision, and M starts at B(w).  The sweeps then only remove what is left
 // of A^{0i}; the frame they converge to is the same.
-template <typename R>
+// N = 4: the general problem.  N = 3: a 2-D event's tensor, whose eta
+// components T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} are zero (the
+// hypertrig rows carry tanh(0) = 0, src/LandauMatch.cpp:37-55); the eta row
+// and column of A and M then never change, so dropping them gives the same
+// frame with fewer operations and registers (t[3], t[6], t[8], t[9] are not
+// read; u^eta = 0).
+template <typename R, int N = 4>
 __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4],
                              const R *w = nullptr) {
+  static_assert(N == 3 || N == 4, "Landau frame in 2+1 or 3+1 dimensions");
   const R tau = (R)TAU;
-  R A[4][4], M[4][4];
+  R A[N][N], M[N][N];
   A[0][0] = t[0];
   A[0][1] = A[1][0] = t[1];
   A[0][2] = A[2][0] = t[2];
-  A[0][3] = A[3][0] = tau * (R)t[3];
   A[1][1] = t[4];
   A[1][2] = A[2][1] = t[5];
-  A[1][3] = A[3][1] = tau * (R)t[6];
   A[2][2] = t[7];
-  A[2][3] = A[3][2] = tau * (R)t[8];
-  A[3][3] = tau * tau * (R)t[9];
+  if constexpr (N == 4) {
+    A[0][3] = A[3][0] = tau * (R)t[3];
+    A[1][3] = A[3][1] = tau * (R)t[6];
+    A[2][3] = A[3][2] = tau * (R)t[8];
+    A[3][3] = tau * tau * (R)t[9];
+  }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) M[i][j] = (i == j) ? (R)1 : (R)0;
+    for (int j = 0; j < N; ++j) M[i][j] = (i == j) ? (R)1 : (R)0;
   if (w) {
-    const R v[4] = {(R)0, w[0], w[1], w[2]};
+    const R v[4] = {(R)0, w[0], w[1], N == 4 ? w[2] : (R)0};
     const R g = sqrt((R)1 + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
     const R h = (R)1 / ((R)1 + g);
     // B(w) = [[g, w^T], [w, I + w w^T / (1 + g)]]; B(-w) flips the 0i signs
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < N; ++j)
         M[i][j] = (i == 0 && j == 0) ? g : (i == 0) ? v[j] : (j == 0) ? v[i] : ((i == j ? (R)1 : (R)0) + h * v[i] * v[j]);
-    R C[4][4];  // A B(-w)
+    R C[N][N];  // A B(-w)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < N; ++j) {
         R acc = (R)0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc += A[i][k] * (((k == 0) != (j == 0)) ? -M[k][j] : M[k][j]);
+        for (int k = 0; k < N; ++k) acc += A[i][k] * (((k == 0) != (j == 0)) ? -M[k][j] : M[k][j]);
         C[i][j] = acc;
       }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = i; j < 4; ++j) {
+      for (int j = i; j < N; ++j) {
         R acc = (R)0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc += (((i == 0) != (k == 0)) ? -M[i][k] : M[i][k]) * C[k][j];
+        for (int k = 0; k < N; ++k) acc += (((i == 0) != (k == 0)) ? -M[i][k] : M[i][k]) * C[k][j];
         A[i][j] = A[j][i] = acc;
       }
   }
+  auto off_diag = [&]() {
+    R o = A[0][1] * A[0][1] + A[0][2] * A[0][2];
+    if constexpr (N == 4) o += A[0][3] * A[0][3];
+    return o;
+  };
   bool ok = true, conv = false;
   for (int sweep = 0; sweep < 60 && ok; ++sweep) {
-    const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
+    const R off = off_diag();
     if (off <= tol2 * A[0][0] * A[0][0]) {
       conv = true;
       break;
     }
 #pragma unroll
-    for (int i = 1; i < 4; ++i) {
+    for (int i = 1; i < N; ++i) {
       const R b = A[0][i];
       if (b == (R)0) continue;
       const R s = A[0][0] + A[i][i], d = A[0][0] - A[i][i];
@@ -636,14 +650,14 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
       A[i][i] = (R)0.5 * (sq - d);
       A[0][i] = A[i][0] = (R)0;
 #pragma unroll
-      for (int k = 1; k < 4; ++k) {
+      for (int k = 1; k < N; ++k) {
         if (k == i) continue;
         const R a0 = A[0][k], ai = A[i][k];
         A[0][k] = A[k][0] = ch * a0 + sh * ai;
         A[i][k] = A[k][i] = sh * a0 + ch * ai;
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < N; ++k) {
         const R m0 = M[k][0], mi = M[k][i];
         M[k][0] = ch * m0 - sh * mi;
         M[k][i] = ch * mi - sh * m0;
@@ -651,7 +665,7 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
     }
   }
   if (ok && !conv) {
-    const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
+    const R off = off_diag();
     conv = off <= loose2 * A[0][0] * A[0][0];
   }
   const R e = A[0][0];
@@ -661,7 +675,10 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
     u_out[0] = (float)u0;
     u_out[1] = (float)M[1][0];
     u_out[2] = (float)M[2][0];
-    u_out[3] = (float)(M[3][0] / tau);
+    if constexpr (N == 4)
+      u_out[3] = (float)(M[N - 1][0] / tau);
+    else
+      u_out[3] = 0.0f;
     return 1;
   }
   *e_out = 1.0e-3f;
@@ -674,8 +691,9 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
 // (on sampled cells of a lumpy 2-D and a 3-D event its float results equal
 // those of a 1e-14 criterion; the eigenvalue error is quadratic in the
 // residual).
+template <int N>
 __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4], const double *w = nullptr) {
-  (void)landau_jacobi<double>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out, w);
+  (void)landau_jacobi<double, N>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out, w);
 }
 
 // FP32 stage of the mixed path.  Returns true when FP32 decides the result to
@@ -684,8 +702,9 @@ __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_
 // small difference of large terms (kappa > 16: nearly null tensors in dilute
 // edge cells, where FP32 rounding in T is amplified by kappa).  Those cells
 // are solved again in FP64.
+template <int N>
 __device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float u_out[4], int *st_out) {
-  const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
+  const int st = landau_jacobi<float, N>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
   *st_out = st;
   // First-order sensitivity of e = T_{mu nu} u^mu u^nu to a relative change
   // of every component: kappa = sum |u_mu| |T^{mu nu}| |u_nu| / e (>= u0^2 T^tt / e).
@@ -756,15 +775,16 @@ struct LandauShared {
 // the first threads of the workgroup, so FP64 work is not spread over
 // partly-idle waves; a converged FP32 frame is the FP64 sweeps' starting
 // point (about half the sweeps of a cold start).
+template <int N>
 __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, const float t[10], float *e,
                                    float u[4], LandauShared &S) {
   if (!ev.mixed) {
-    if (live) landau_fp64(t, ev.tau, e, u);
+    if (live) landau_fp64<N>(t, ev.tau, e, u);
     return;
   }
   bool need = false;
   int st = 0;
-  if (live) need = !landau_fp32_ok(t, ev.tau, e, u, &st);
+  if (live) need = !landau_fp32_ok<N>(t, ev.tau, e, u, &st);
   __syncthreads();  // the previous group's list is no longer read
   if (threadIdx.x == 0) S.n64 = 0;
   __syncthreads();
@@ -786,7 +806,7 @@ __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, 
     for (int c = 0; c < 10; ++c) tt[c] = ev.T[c * ev.t_plane + ti];
     // the FP32 frame (orthonormal components) seeds the FP64 sweeps
     const double w3[3] = {(double)S.u[1][w], (double)S.u[2][w], (double)ev.tau * (double)S.u[3][w]};
-    landau_fp64(tt, ev.tau, &e64, u64, S.warm[w] ? w3 : nullptr);
+    landau_fp64<N>(tt, ev.tau, &e64, u64, S.warm[w] ? w3 : nullptr);
     S.e[w] = e64;
 #pragma unroll
     for (int c = 0; c < 4; ++c) S.u[c][w] = u64[c];
@@ -801,13 +821,14 @@ __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, 
 
 // Landau frame, EoS, bulk and shear parts of the cell whose T components sit
 // at index tidx of ev.T's planes (workgroup-uniform call, see above).
+template <int N>
 __device__ void landau_cell_compute(const EventLandau &ev, bool live, long tidx, CellOut &r, LandauShared &S) {
 #pragma clang fp contract(off)
   float t[10];
 #pragma unroll
-  for (int c = 0; c < 10; ++c) t[c] = live ? ev.T[c * ev.t_plane + tidx] : 0.0f;
+  for (int c = 0; c < 10; ++c) t[c] = (live && (N == 4 || (c != 3 && c != 6 && c != 8 && c != 9))) ? ev.T[c * ev.t_plane + tidx] : 0.0f;
   float e = 0.0f, u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  landau_group_solve(ev, live, tidx, t, &e, u, S);
+  landau_group_solve<N>(ev, live, tidx, t, &e, u, S);
   if (!live) return;
   const float TAU = ev.tau;
   const float p = eos_pressure(e, ev.eos_type);
@@ -886,6 +907,7 @@ __device__ void cell_store_float(const EventLandau &ev, long is, const CellOut &
 
 // One cell per thread, reference index is (T read through the layout map);
 // workgroup-uniform call.
+template <int N>
 __device__ void landau_cell_out(const EventLandau &ev, bool live, long is, LandauShared &S) {
   long tidx = is;
   if (ev.t_eta_major) {
@@ -894,7 +916,7 @@ __device__ void landau_cell_out(const EventLandau &ev, bool live, long is, Landa
     tidx = ie * ev.nxy + xy;
   }
   CellOut r;
-  landau_cell_compute(ev, live, tidx, r, S);
+  landau_cell_compute<N>(ev, live, tidx, r, S);
   if (!live) return;
   if (ev.out) {
 #pragma unroll
@@ -919,7 +941,7 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds,
   const bool live = xy < ev.nxy && ie < ev.neta;
   const int np = ev.J ? FSM_NUM_OUTPUTS + 5 : FSM_NUM_OUTPUTS;
   CellOut r;
-  landau_cell_compute(ev, live, (long)ie * ev.nxy + xy, r, S);  // starts with a barrier
+  landau_cell_compute<4>(ev, live, (long)ie * ev.nxy + xy, r, S);  // starts with a barrier
   if (live) {
     const int li = xl * FSML_PITCH + el;
 #pragma unroll
@@ -941,12 +963,13 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds,
   }
 }
 
+template <int N>
 __global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
   __shared__ LandauShared S;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long base = (long)blockIdx.x * blockDim.x; base < ev.dim; base += stride) {
     const long is = base + threadIdx.x;
-    landau_cell_out(ev, is < ev.dim, is, S);
+    landau_cell_out<N>(ev, is < ev.dim, is, S);
   }
 }
 
@@ -971,8 +994,22 @@ __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restr
       landau3d_tile(ev, item.y, item.z, lds_out, S);
     } else {
       for (int base = 0; base < item.z; base += blockDim.x)
-        landau_cell_out(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
+        landau_cell_out<4>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
     }
+  }
+}
+
+// Batched 2-D events whose tensors come from the 2-D stream kernel (eta
+// components exactly zero): item = (event, cell offset, cell count, 0), the
+// 3x3 Landau frame.
+__global__ __launch_bounds__(256) void k_landau2d_batch(const EventLandau *__restrict__ evs,
+                                                        const int4 *__restrict__ items, int n_items) {
+  __shared__ LandauShared S;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    int4 item = items[xcd_order(it, n_items)];
+    const EventLandau ev = evs[item.x];
+    for (int base = 0; base < item.z; base += blockDim.x)
+      landau_cell_out<3>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
   }
 }
 
@@ -1021,7 +1058,10 @@ hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s) {
     return hipGetLastError();
   }
   int blocks = (int)std::min<long>((ev.dim + 255) / 256, 65535L);
-  hipLaunchKernelGGL(k_landau_one, dim3(blocks), dim3(256), 0, s, ev);
+  if (ev.eta_zero)
+    hipLaunchKernelGGL(k_landau_one<3>, dim3(blocks), dim3(256), 0, s, ev);
+  else
+    hipLaunchKernelGGL(k_landau_one<4>, dim3(blocks), dim3(256), 0, s, ev);
   return hipGetLastError();
 }
 
@@ -1070,5 +1110,11 @@ hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, in
                                    hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_landau_batch, dim3(batch_grid(n)), dim3(256), lds_bytes, s, evs, items, n);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_landau2d_batch(const EventLandau *evs, const int4 *items, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_landau2d_batch, dim3(batch_grid(n)), dim3(256), 0, s, evs, items, n);
   return hipGetLastError();
 }
