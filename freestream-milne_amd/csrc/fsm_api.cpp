@@ -439,6 +439,17 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     if (!sample_offset_exact(st[j], nx, ny, xmin, ymin, p.DX, p.DY)) amb.push_back(j);
   FSM_TRY(pl.gx.upload(gxw));
   FSM_TRY(pl.gy.upload(gyw));
+  // interior rows (columns) whose band weights are bitwise those of the middle row
+  auto canon = [](const std::vector<float> &g, int n, int *lo, int *hi) {
+    const int W = FSM_SPLINE_W, c = n / 2;
+    auto same = [&](int i) { return memcmp(&g[(size_t)i * W], &g[(size_t)c * W], W * sizeof(float)) == 0; };
+    *lo = *hi = c;
+    while (*lo > 0 && same(*lo - 1)) --*lo;
+    while (*hi < n - 1 && same(*hi + 1)) ++*hi;
+  };
+  int cxlo, cxhi, cylo, cyhi;
+  canon(gxw, nx, &cxlo, &cxhi);
+  canon(gyw, ny, &cylo, &cyhi);
   FSM_TRY(pl.st.upload(st));
   FSM_TRY(pl.groups.upload(groups));
   FSM_TRY(pl.planes.ensure((size_t)nx * ny));
@@ -478,6 +489,10 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.planes = pl.planes.p;
   ev.gx = pl.gx.p;
   ev.gy = pl.gy.p;
+  ev.cxlo = cxlo;
+  ev.cxhi = cxhi;
+  ev.cylo = cylo;
+  ev.cyhi = cyhi;
   ev.st = pl.st.p;
   ev.groups = pl.groups.p;
   ev.t_plane = p.DIM;
@@ -1485,12 +1500,12 @@ int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p) {
     std::vector<int4> ks, sp, s2;
     size_t lds = 0;
     char *cur = ws.p;
-    const int stx = (nx + FSMSP_TX - 1) / FSMSP_TX, sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
+    const int sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
     for (int i = 0; i < n; ++i) {
       Event2D d = event2d_in_workspace(pl, p, d_in.p, path, v, cur);
       if (d.mfma)
         for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
-      for (int x = 0; x < stx; ++x)
+      for (int x = 0; x < (nx + FSMSP2_TX - 1) / FSMSP2_TX; ++x)
         for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
       for (int y = 0; y < d.tiles_y; ++y)
         for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));
@@ -1630,8 +1645,8 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       if (d.mfma)
         for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
       e2.push_back(d);
-      const int stx = (p.DIM_X + FSMSP_TX - 1) / FSMSP_TX, sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
-      for (int x = 0; x < stx; ++x)
+      const int sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
+      for (int x = 0; x < (p.DIM_X + FSMSP2_TX - 1) / FSMSP2_TX; ++x)
         for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
       for (int y = 0; y < d.tiles_y; ++y)
         for (int x = 0; x < d.tiles_x; ++x) s2.push_back(make_int4(i, x, y, 0));

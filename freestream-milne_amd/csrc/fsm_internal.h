@@ -68,6 +68,8 @@ __device__ __forceinline__ int xcd_order(int b, int n) {
 #define FSM_SPLINE_W (2 * FSM_SPLINE_BAND + 1)
 #define FSMSP_TX 16
 #define FSMSP_TY 32
+// Batched spline tiles: FSMSP2_TX rows x FSMSP_TY columns (several flag tiles).
+#define FSMSP2_TX 64
 
 // One azimuthal sample of the 2-D stencil.  For a fixed phi_j the source
 // point x - DTAU cos(phi_j) is the same shift for every cell
@@ -109,6 +111,8 @@ struct Event2D {
   float4 *planes;       // spline planes (written by the spline kernel)
   const float *gx;      // [nx][FSM_SPLINE_W] derivative band weights along x
   const float *gy;      // [ny][FSM_SPLINE_W]
+  int cxlo, cxhi;       // rows [cxlo, cxhi] of gx are bitwise the same weights (away from the edges)
+  int cylo, cyhi;       // columns [cylo, cyhi] of gy likewise
   const Stencil2D *st;  // [DIM_PHIP]
   const StencilGroup *groups;
   float *T;             // 10 planes, reference layout

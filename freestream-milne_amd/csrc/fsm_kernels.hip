@@ -172,6 +172,181 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
   }
 }
 
+// Batched spline tile: FSMSP2_TX rows x FSMSP_TY columns of knots, with the
+// same outputs as spline2d_tile bit for bit (each slope is the same
+// sequential fmaf chain over the same band weights, and each FSMSP_TX x
+// FSMSP_TY flag tile keeps its own positivity tolerance) but with the band
+// products from registers: for z_x a lane owns one column and 16 rows and
+// slides a 48-value window down the column; for z_y and z_xy a lane owns one
+// row and 8 columns and slides a 40-value window along the row.  A wave's
+// rows (z_x) or columns (z_y) are uniform, so the weights are scalar loads,
+// one set for the whole wave where its rows or columns all carry the
+// interior weights ([cxlo, cxhi], [cylo, cyhi]).
+#define SP2_RX (FSMSP2_TX + 2 * FSM_SPLINE_BAND)
+#define SP2_RY (FSMSP_TY + 2 * FSM_SPLINE_BAND)
+#define SP2_P (SP2_RY + 1)  // odd pitch: lanes walking rows do not share banks
+// The band weights are read-only for the kernel's lifetime: read through the
+// constant address space, so uniform reads become scalar loads.
+typedef __attribute__((address_space(4))) const float fsm_const_f;
+
+struct SplineLds2 {
+  float zr[SP2_RX * SP2_P];      // z over the tile and the band-wide halo
+  float zxr[FSMSP2_TX * SP2_P];  // z_x over the tile rows and the y halo
+  float zmax[4][FSMSP2_TX / FSMSP_TX];
+  unsigned bad[FSMSP2_TX / FSMSP_TX];
+  float sc;
+};
+
+__device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L) {
+  constexpr int B = FSM_SPLINE_BAND, W = FSM_SPLINE_W, NF = FSMSP2_TX / FSMSP_TX;
+  constexpr int RA = FSMSP2_TX / 4;       // z_x rows per wave
+  constexpr int CB = FSMSP_TY / 4;        // z_y columns per wave
+  constexpr int NS = SP2_RX / 4;          // staged rows per wave
+  static_assert(SP2_RY == 64 && FSMSP2_TX == 64, "one lane per staged column and per tile row");
+  const int nx = ev.nx, ny = ev.ny;
+  const int x0 = tx2 * FSMSP2_TX, y0 = ty * FSMSP_TY;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  __syncthreads();  // the previous item's reads of L are done
+  // z over staged rows s = wave + 4 u (x = x0 - B + s), lane = column (y = y0 - B + lane)
+  {
+    const int gy = y0 - B + lane;
+    const bool yin = gy >= 0 && gy < ny;
+    float z[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int gx = x0 - B + wave + 4 * u;
+      z[u] = (yin && gx >= 0 && gx < nx) ? knot_z(ev, ev.e_gev[(long)gx * ny + gy]) : 0.0f;
+    }
+    // largest |z| over each flag tile's band region (staged rows [16 i, 16 i + 47])
+    float zm[NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) zm[i] = 0.0f;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int srow = wave + 4 * u;
+      L.zr[srow * SP2_P + lane] = z[u];
+#pragma unroll
+      for (int i = 0; i < NF; ++i)
+        if (srow >= FSMSP_TX * i && srow < FSMSP_TX * i + FSMSP_TX + 2 * B) zm[i] = fmaxf(zm[i], fabsf(z[u]));
+    }
+    if (ev.planes_h) {
+#pragma unroll
+      for (int i = 0; i < NF; ++i) {
+        float m = zm[i];
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0) L.zmax[wave][i] = m;
+      }
+      if (threadIdx.x < 64) {
+        const float sc = knot_scale_from_parts(ev);
+        if (threadIdx.x == 0) {
+          L.sc = sc;
+          ev.mscale[FSM_KS_PARTS] = sc;  // the same value from every block
+        }
+      }
+      if (threadIdx.x < NF) L.bad[threadIdx.x] = 0u;
+    }
+  }
+  __syncthreads();
+  // z_x (grid units) for tile rows x0 + RA wave + r and staged column lane
+  {
+    const int gy = y0 - B + lane;
+    const bool yin = gy >= 0 && gy < ny;
+    float win[RA + 2 * B];
+#pragma unroll
+    for (int u = 0; u < RA + 2 * B; ++u) win[u] = L.zr[(RA * wave + u) * SP2_P + lane];
+    const int r0 = x0 + RA * wave;
+    if (r0 >= ev.cxlo && r0 + RA - 1 <= ev.cxhi) {
+      fsm_const_f *w = (fsm_const_f *)ev.gx + (long)ev.cxlo * W;  // every row of this wave has these weights
+#pragma unroll
+      for (int r = 0; r < RA; ++r) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int m = 0; m < W; ++m) acc = fmaf(w[m], win[r + m], acc);
+        L.zxr[(RA * wave + r) * SP2_P + lane] = (yin && r0 + r < nx) ? acc : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RA; ++r) {
+        float acc = 0.0f;
+        if (r0 + r < nx) {
+          fsm_const_f *w = (fsm_const_f *)ev.gx + (long)(r0 + r) * W;
+#pragma unroll
+          for (int m = 0; m < W; ++m) acc = fmaf(w[m], win[r + m], acc);
+        }
+        L.zxr[(RA * wave + r) * SP2_P + lane] = (yin && r0 + r < nx) ? acc : 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+  // z_y, z_xy for tile row lane (x = x0 + lane) and columns y0 + CB wave + j
+  {
+    float zw[CB + 2 * B], zxw[CB + 2 * B];
+#pragma unroll
+    for (int u = 0; u < CB + 2 * B; ++u) {
+      zw[u] = L.zr[(lane + B) * SP2_P + CB * wave + u];
+      zxw[u] = L.zxr[lane * SP2_P + CB * wave + u];
+    }
+    const int gx = x0 + lane;
+    const int c0 = y0 + CB * wave;
+    const bool canon = c0 >= ev.cylo && c0 + CB - 1 <= ev.cyhi;
+    const int fi = lane / FSMSP_TX;  // this row's flag tile
+    float tol = 0.0f, sc = 0.0f;
+    if (ev.planes_h) {
+      sc = L.sc;
+      float zmx = 0.0f;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+          if (i == fi) zmx = fmaxf(zmx, L.zmax[v][i]);
+      tol = -FSM_MFMA_POS_EPS * zmx;
+    }
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int gy = c0 + j;
+      if (gy >= ny) break;
+      fsm_const_f *w = (fsm_const_f *)ev.gy + (long)(canon ? ev.cylo : gy) * W;
+      float zy = 0.0f, zxy = 0.0f;
+#pragma unroll
+      for (int m = 0; m < W; ++m) {
+        zy = fmaf(w[m], zw[j + m], zy);
+        zxy = fmaf(w[m], zxw[j + m], zxy);
+      }
+      if (gx < nx) {
+        const float z = zw[j + B], zx = zxw[j + B];
+        ev.planes[(long)gx * ny + gy] = make_float4(z, zy, zx, zxy);
+        if (ev.planes_h) {
+          const float big = fmaxf(fmaxf(fabsf(z), fabsf(zy)), fmaxf(fabsf(zx), fabsf(zxy))) * sc;
+          bad |= !(knot_cp_min(z, zy, zx, zxy) >= tol) || !(big <= FSM_MFMA_MAXABS);
+          unsigned short h[4], l[4];
+          split_half(z * sc, h[0], l[0]);
+          split_half(zy * sc, h[1], l[1]);
+          split_half(zx * sc, h[2], l[2]);
+          split_half(zxy * sc, h[3], l[3]);
+          ev.planes_h[(long)gx * ny + gy] =
+              make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
+                         l[0] | ((unsigned)l[1] << 16), l[2] | ((unsigned)l[3] << 16));
+        }
+      }
+    }
+    if (ev.planes_h) {
+      const unsigned long long bm = __ballot(bad);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+          if ((bm >> (FSMSP_TX * i)) & 0xFFFFull) atomicOr(&L.bad[i], 1u);
+      }
+    }
+  }
+  if (ev.planes_h) {
+    __syncthreads();
+    const int sx = tx2 * NF + (int)threadIdx.x;
+    if ((int)threadIdx.x < NF && sx * FSMSP_TX < nx) ev.sp_flags[sx * ev.sp_tiles_y + ty] = L.bad[threadIdx.x];
+  }
+}
+
 // Knot scale of the MFMA form: 2^k <= FSM_MFMA_ZTARGET / max|z| over the
 // event (z as the spline kernel computes it).  Pass 1 (k_knot_scale): part p
 // of FSM_KS_PARTS writes the largest |z| of its contiguous chunk to
@@ -240,13 +415,14 @@ __global__ __launch_bounds__(256) void k_spline2d_one(Event2D ev) {
   spline2d_tile(ev, t / tiles_y, t % tiles_y, L);
 }
 
+// Batched: item = (event, FSMSP2_TX-row tile, FSMSP_TY-column tile).
 __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restrict__ evs,
                                                         const int4 *__restrict__ items, int n_items) {
-  __shared__ SplineLds L;
+  __shared__ SplineLds2 L;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     int4 item = items[xcd_order(it, n_items)];
     const Event2D ev = evs[item.x];
-    spline2d_tile(ev, item.y, item.z, L);
+    spline2d_tile2(ev, item.y, item.z, L);
   }
 }
 
