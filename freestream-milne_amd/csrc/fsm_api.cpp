@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -265,7 +266,9 @@ unsigned short half_bits(double v) {
 // order.  Offset record: (ox, oy, LDS delta in the group's box, 0).
 struct MfmaTables {
   std::vector<int4> offs;
-  std::vector<uint4> bmat;
+  std::vector<int4> recs;    // per group: its offsets' records, then its pair-MFMA records
+  std::vector<uint4> bmat;   // per group, in the order of recs: 64 lanes per MFMA
+  int npairs = 0;            // pair MFMAs of all groups (interior form)
   std::vector<StencilGroup> groups;
   int lds4 = 0;
   int rbx0 = 0, rbx1 = 0, rby0 = 0, rby1 = 0;
@@ -297,29 +300,83 @@ void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char>
     }
   }
   const size_t no = t.offs.size();
-  t.bmat.assign(no * 64, make_uint4(0, 0, 0, 0));
-  for (size_t o = 0; o < no; ++o)
+  // B operand of one MFMA from its 32x6 weights wt(g, c, m) (corner/lane
+  // group g, knot component c, moment m): columns 0-5 hold W_h for both the
+  // hi and lo knot halves, columns 8-13 W_l for the hi half.  Stored packed:
+  // lane l's four distinct halves (W_h or W_l of its column for c = 0..3),
+  // 8 bytes, 64 lanes in 32 uint4; the kernel repeats them in the upper
+  // halves for columns 0-5 (fsm_stream2d.hip, bmat_expand).
+  auto bmat_lanes = [](auto wt, uint4 *out) {
+    unsigned packed[128];
     for (int lane = 0; lane < 64; ++lane) {
       const int n = lane & 15, g = lane >> 4;
-      unsigned short h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      unsigned short h[4] = {0, 0, 0, 0};
       for (int c = 0; c < 4; ++c) {
         if (n < 6) {
-          const double w = W[o][(g * 4 + c) * 6 + n];
-          h[c] = h[4 + c] = half_bits(w);
+          h[c] = half_bits(wt(g, c, n));
         } else if (n >= 8 && n < 14) {
-          const double w = W[o][(g * 4 + c) * 6 + n - 8];
+          const double w = wt(g, c, n - 8);
           const _Float16 hi = (_Float16)w;
           h[c] = half_bits(w - (double)hi);
         }
       }
-      t.bmat[o * 64 + lane] = make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
-                                         h[4] | ((unsigned)h[5] << 16), h[6] | ((unsigned)h[7] << 16));
+      packed[2 * lane] = h[0] | ((unsigned)h[1] << 16);
+      packed[2 * lane + 1] = h[2] | ((unsigned)h[3] << 16);
     }
-  // greedy groups along the phi order of first appearance
+    memcpy(out, packed, sizeof(packed));
+  };
+  // Interior form: the knot displacements d = o + corner of offsets [o0, o1)
+  // (weights summed per d), covered row by row by dominoes (dx, dx+1) inside
+  // the box [bx0, bx1] (a displacement at bx1 with no partner takes the
+  // domino (bx1-1, bx1)); each displacement's weights go to exactly one half.
+  struct Domino {
+    int dx, dy;
+    std::array<double, 24> v[2];  // halves gx = 0, 1: [c * 6 + m]
+  };
+  auto cover = [&](size_t o0, size_t o1, int bx1, std::vector<Domino> *out) {
+    std::map<std::pair<int, int>, std::array<double, 24>> V;  // (dy, dx) -> weights
+    for (size_t o = o0; o < o1; ++o)
+      for (int g = 0; g < 4; ++g) {
+        auto &v = V[{t.offs[o].y + (g >> 1), t.offs[o].x + (g & 1)}];
+        for (int k = 0; k < 24; ++k) v[k] += W[o][g * 24 + k];
+      }
+    int count = 0;
+    for (auto it = V.begin(); it != V.end();) {
+      Domino D;
+      D.dy = it->first.first;
+      const int d = it->first.second;
+      D.v[0].fill(0.0);
+      D.v[1].fill(0.0);
+      if (d + 1 <= bx1) {
+        D.dx = d;
+        D.v[0] = it->second;
+        ++it;
+        if (it != V.end() && it->first.first == D.dy && it->first.second == d + 1) {
+          D.v[1] = it->second;
+          ++it;
+        }
+      } else {
+        D.dx = d - 1;
+        D.v[1] = it->second;
+        ++it;
+      }
+      ++count;
+      if (out) out->push_back(D);
+    }
+    return count;
+  };
+  std::vector<std::vector<uint4>> offb(no, std::vector<uint4>(FSM_BMAT_U4));
+  for (size_t o = 0; o < no; ++o)
+    bmat_lanes([&](int g, int c, int m) { return W[o][(g * 4 + c) * 6 + m]; }, offb[o].data());
+  // greedy groups along the phi order of first appearance; a group's LDS
+  // image is its knot box, one packed B operand (64 x 8 B) and one record
+  // (16 B) per MFMA of either form
   const int TX = fsm_stream2d_tile_w(0), TY = fsm_stream2d_tile_h(0);
   t.lds4 = 0;
   t.rbx0 = t.rby0 = 1 << 30;
   t.rbx1 = t.rby1 = -(1 << 30);
+  t.bmat.clear();
+  std::vector<int4> recs;
   size_t o = 0;
   while (o < no) {
     StencilGroup G{};
@@ -332,25 +389,60 @@ void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char>
     for (; e < no; ++e) {
       const int bx0 = std::min(G.bx0, t.offs[e].x), bx1 = std::max(G.bx1, t.offs[e].x + 1);
       const int by0 = std::min(G.by0, t.offs[e].y), by1 = std::max(G.by1, t.offs[e].y + 1);
-      if ((TX + bx1 - bx0) * ((TY + by1 - by0 + 15) & ~15) + 65 * (int)(e + 1 - o) > budget4) break;
+      const int np = (cover(o, e + 1, bx1, nullptr) + 1) / 2;
+      if ((TX + bx1 - bx0) * ((TY + by1 - by0 + 15) & ~15) + (FSM_BMAT_U4 + 1) * ((int)(e + 1 - o) + np) >
+          budget4)
+        break;
       G.bx0 = bx0;
       G.bx1 = bx1;
       G.by0 = by0;
       G.by1 = by1;
     }
     G.j1 = (int)e;
+    std::vector<Domino> dom;
+    cover(o, e, G.bx1, &dom);
+    const int np = ((int)dom.size() + 1) / 2;
     // knot box columns padded to 16 records (TY rows + halo, rounded up)
     const int LYP = (TY + G.by1 - G.by0 + 15) & ~15;
-    t.lds4 = std::max(t.lds4, (TX + G.bx1 - G.bx0) * LYP + 65 * (G.j1 - G.j0));
-    // LDS delta of each offset inside the group's knot box
-    for (size_t k = o; k < e; ++k) t.offs[k].z = (t.offs[k].x - G.bx0) * LYP + (t.offs[k].y - G.by0);
+    t.lds4 = std::max(t.lds4, (TX + G.bx1 - G.bx0) * LYP + (FSM_BMAT_U4 + 1) * (G.j1 - G.j0 + np));
+    // the group's block: its offsets' records and B operands, then its pairs'
+    G.pad0 = (int)recs.size();
+    G.pad1 = np;
+    for (size_t k = o; k < e; ++k) {
+      // LDS delta of each offset inside the group's knot box
+      t.offs[k].z = (t.offs[k].x - G.bx0) * LYP + (t.offs[k].y - G.by0);
+      recs.push_back(t.offs[k]);
+      t.bmat.insert(t.bmat.end(), offb[k].begin(), offb[k].end());
+    }
+    for (int q = 0; q < np; ++q) {
+      // lane groups gy = 0, 1 take dominoes 2q, 2q+1 (an odd count leaves the
+      // second half empty, reading the first domino's knots with zero weights)
+      const Domino &A = dom[2 * q];
+      const bool two = 2 * q + 1 < (int)dom.size();
+      const Domino &Bd = two ? dom[2 * q + 1] : A;
+      const int da = (A.dx - G.bx0) * LYP + (A.dy - G.by0), db = (Bd.dx - G.bx0) * LYP + (Bd.dy - G.by0);
+      recs.push_back(make_int4(da, db, 0, 0));
+      uint4 lanes[FSM_BMAT_U4];
+      bmat_lanes(
+          [&](int g, int c, int m) {
+            const int gx = g & 1, gy = g >> 1;
+            if (gy && !two) return 0.0;
+            return (gy ? Bd : A).v[gx][c * 6 + m];
+          },
+          lanes);
+      t.bmat.insert(t.bmat.end(), lanes, lanes + FSM_BMAT_U4);
+    }
     t.rbx0 = std::min(t.rbx0, G.bx0);
     t.rbx1 = std::max(t.rbx1, G.bx1);
     t.rby0 = std::min(t.rby0, G.by0);
     t.rby1 = std::max(t.rby1, G.by1);
     t.groups.push_back(G);
+    t.npairs += np;
     o = e;
   }
+  t.recs = recs;
+  if (t.bmat.empty()) t.bmat.assign(FSM_BMAT_U4, make_uint4(0, 0, 0, 0));
+  if (t.recs.empty()) t.recs.push_back(make_int4(0, 0, 0, 0));
   if (no == 0) t.rbx0 = t.rbx1 = t.rby0 = t.rby1 = 0;
 }
 
@@ -453,7 +545,7 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   FSM_TRY(pl.st.upload(st));
   FSM_TRY(pl.groups.upload(groups));
   FSM_TRY(pl.planes.ensure((size_t)nx * ny));
-  FSM_TRY(pl.moffs.upload(mt.offs));
+  FSM_TRY(pl.moffs.upload(mt.recs));
   FSM_TRY(pl.bmat.upload(mt.bmat));
   FSM_TRY(pl.mgroups.upload(mt.groups));
   FSM_TRY(pl.amb.upload(amb));

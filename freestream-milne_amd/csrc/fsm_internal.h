@@ -129,7 +129,7 @@ struct Event2D {
   unsigned *sp_flags;   // per spline tile: nonzero if a knot is unsafe for the MFMA form
   float *mscale;        // per event: FSM_KS_PARTS partial max |z|, then the power-of-two knot scale
   const int4 *moffs;    // [n_moff] (ox, oy, LDS delta of the offset in its group's knot box, -)
-  const uint4 *bmat;    // [n_moff][64] MFMA B operand per lane (8 halves)
+  const uint4 *bmat;    // per MFMA FSM_BMAT_U4 uint4: the B operand packed, 4 halves per lane
   const StencilGroup *mgroups;
   const int *amb;       // [n_amb] indices into st
   // ---- charge pass of the baryon extension ----
@@ -155,6 +155,8 @@ struct Event2D {
 // (Bernstein control points, where the reference falls back to bilinear),
 // and MFMA tiles that read a flagged knot run the FP32 sample loop instead.
 #define FSM_MFMA_ZTARGET 4096.0f
+// 16-byte units of one packed MFMA B operand (64 lanes x 8 bytes)
+#define FSM_BMAT_U4 32
 #define FSM_KS_PARTS 64
 #define FSM_MFMA_MAXABS 30000.0f
 #define FSM_MFMA_POS_EPS 1.0e-5f

@@ -286,6 +286,19 @@ __device__ __forceinline__ float sample_correction(const Event2D &ev, const Sten
   return (ok ? exact : 0.0f) - (idx ? v : 0.0f);
 }
 
+// B operands are stored packed: lane l's four distinct halves (8 bytes).
+// Columns 0-5 (lane % 16 < 6) repeat them in halves 4-7, the W_h of both knot
+// halves; the other columns keep halves 4-7 zero.  `b` is the operand for
+// cell columns sharing the lane's corner roles, `bs` for the odd columns
+// (B'[lane] = B[lane ^ 16]: ds_swizzle bit mode and 0x1f, xor 0x10).
+__device__ __forceinline__ void bmat_expand(const uint4 *bl, int m, int lane, bool dup, fsm_half8 &b, fsm_half8 &bs) {
+  const uint2 v = reinterpret_cast<const uint2 *>(bl + m * FSM_BMAT_U4)[lane];
+  const uint2 w = make_uint2((unsigned)__builtin_amdgcn_ds_swizzle((int)v.x, 0x401F),
+                             (unsigned)__builtin_amdgcn_ds_swizzle((int)v.y, 0x401F));
+  b = __builtin_bit_cast(fsm_half8, make_uint4(v.x, v.y, dup ? v.x : 0u, dup ? v.y : 0u));
+  bs = __builtin_bit_cast(fsm_half8, make_uint4(w.x, w.y, dup ? w.x : 0u, dup ? w.y : 0u));
+}
+
 // The MFMA loop over the knot offsets of one LDS stage, for the wave's XW
 // cell columns x 2 row blocks.  Adjacent cell columns share a knot column:
 // register R[k] holds knot column 2k (lanes with corner-x 0) and 2k+1 (corner
@@ -304,24 +317,13 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
                                              int xw0, int y0, int nx, int ny) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
-  // the offset record and weights of the next offset are read one iteration
-  // ahead, off the critical path of the knot loads
-  int4 rec_n = orec[0];
-  uint4 bw_n = bl[lane];
+  const bool dup = (lane & 15) < 6;
   for (int o = 0; o < no; ++o) {
-    const int4 rec = rec_n;
-    const uint4 bw = bw_n;
-    if (o + 1 < no) {
-      rec_n = orec[o + 1];
-      bw_n = bl[(o + 1) * 64 + lane];
-    }
+    const int4 rec = orec[o];
     const int ox = __builtin_amdgcn_readfirstlane(rec.x), oy = __builtin_amdgcn_readfirstlane(rec.y);
     const int delta = __builtin_amdgcn_readfirstlane(rec.z);
-    const fsm_half8 b0 = __builtin_bit_cast(fsm_half8, bw);
-    // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 -> lane ^ 16 within 32 lanes
-    const uint4 bs = make_uint4(__builtin_amdgcn_ds_swizzle((int)bw.x, 0x401F), __builtin_amdgcn_ds_swizzle((int)bw.y, 0x401F),
-                                __builtin_amdgcn_ds_swizzle((int)bw.z, 0x401F), __builtin_amdgcn_ds_swizzle((int)bw.w, 0x401F));
-    const fsm_half8 b1 = __builtin_bit_cast(fsm_half8, bs);
+    fsm_half8 b0, b1;
+    bmat_expand(bl, o, lane, dup, b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
       // one 16-row block at a time: the knot registers of a block are
@@ -354,6 +356,49 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
         uint4 a = R[xc / 2][0];
         if (xc & 1) {
           const uint4 n1 = R[xc / 2 + 1][0];
+          a.x = gx ? a.x : n1.x;
+          a.y = gx ? a.y : n1.y;
+          a.z = gx ? a.z : n1.z;
+          a.w = gx ? a.w : n1.w;
+        }
+        acc[xc][yb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(fsm_half8, a), (xc & 1) ? b1 : b0,
+                                                             acc[xc][yb], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Interior form of one LDS stage: the group's knot displacements (offset +
+// corner, weights summed per displacement) covered by dominoes (dx, dx+1) at
+// one row dy; one MFMA takes two dominoes, lane group (gx, gy) reading knot
+// (x + dx_gy + gx, y + dy_gy), so only the row offset differs between the gy
+// halves (a per-lane box delta).  The column sharing of mfma_offsets is
+// unchanged.  Used where every cell of the wave keeps every sample of the
+// group (no masks), so merging samples of different offsets is exact.
+template <int XW>
+__device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
+                                           const int4 *prec, int np, int lane_base, int LYP, int gx, int gy) {
+  const int lane = threadIdx.x & 63;
+  constexpr int NR = XW / 2 + 1;
+  const bool dup = (lane & 15) < 6;
+  for (int o = 0; o < np; ++o) {
+    const int4 rec = prec[o];
+    const int delta = gy ? rec.y : rec.x;
+    fsm_half8 b0, b1;
+    bmat_expand(bl, o, lane, dup, b0, b1);
+#pragma unroll
+    for (int yb = 0; yb < 2; ++yb) {
+      uint4 R[NR];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        const int col = (k < XW / 2 || XW == 1) ? lane_base : lane_base - gx * LYP;
+        R[k] = lh[col + delta + 2 * k * LYP + yb * 16];
+      }
+#pragma unroll
+      for (int xc = 0; xc < XW; ++xc) {
+        uint4 a = R[xc / 2];
+        if (xc & 1) {
+          const uint4 n1 = R[xc / 2 + 1];
           a.x = gx ? a.x : n1.x;
           a.y = gx ? a.y : n1.y;
           a.z = gx ? a.z : n1.z;
@@ -412,21 +457,23 @@ __device__ __forceinline__ void group_load(GroupPrefetch &pf, const Event2D &ev,
     if (k < n && (unsigned)kx < (unsigned)nx && (unsigned)ky < (unsigned)ny) pf.box[u] = ev.planes_h[(long)kx * ny + ky];
     wk.next();
   }
-  const int nw = (G.j1 - G.j0) * 64;
+  // the group's block (G.pad0): its offsets' records and B operands, then
+  // its G.pad1 pair MFMAs'
+  const int nm = G.j1 - G.j0 + G.pad1, nw = nm * FSM_BMAT_U4;
 #pragma unroll
   for (int u = 0; u < kPfW; ++u) {
     const int k = threadIdx.x + u * blockDim.x;
-    if (k < nw) pf.w[u] = ev.bmat[(long)G.j0 * 64 + k];
+    if (k < nw) pf.w[u] = ev.bmat[(long)G.pad0 * FSM_BMAT_U4 + k];
   }
-  if ((int)threadIdx.x < G.j1 - G.j0) pf.rec = ev.moffs[G.j0 + threadIdx.x];
+  if ((int)threadIdx.x < nm) pf.rec = ev.moffs[G.pad0 + threadIdx.x];
 }
 
 __device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event2D &ev, const StencilGroup &G,
                                             uint4 *lh, int x0, int y0, int TX, int TY) {
   const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
-  const int no = G.j1 - G.j0, n = LX * LY, step = blockDim.x;
+  const int no = G.j1 - G.j0 + G.pad1, n = LX * LY, step = blockDim.x;  // MFMAs of both forms
   uint4 *bl = lh + LX * LYP;
-  int4 *orec = reinterpret_cast<int4 *>(bl + no * 64);
+  int4 *orec = reinterpret_cast<int4 *>(bl + no * FSM_BMAT_U4);
   BoxWalk wk(LY);
 #pragma unroll
   for (int u = 0; u < kPfBox; ++u) {
@@ -440,13 +487,13 @@ __device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event
     if ((unsigned)kx < (unsigned)ev.nx && (unsigned)ky < (unsigned)ev.ny) v = ev.planes_h[(long)kx * ev.ny + ky];
     lh[lx * LYP + ly] = v;
   }
-  const int nw = no * 64;
+  const int nw = no * FSM_BMAT_U4;
 #pragma unroll
   for (int u = 0; u < kPfW; ++u) {
     const int k = threadIdx.x + u * step;
     if (k < nw) bl[k] = pf.w[u];
   }
-  for (int k = threadIdx.x + kPfW * step; k < nw; k += step) bl[k] = ev.bmat[(long)G.j0 * 64 + k];
+  for (int k = threadIdx.x + kPfW * step; k < nw; k += step) bl[k] = ev.bmat[(long)G.pad0 * FSM_BMAT_U4 + k];
   if ((int)threadIdx.x < no) orec[threadIdx.x] = pf.rec;
 }
 
@@ -491,9 +538,9 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     // box columns are padded to 16 records (256 B) so that the lanes reading
     // knot columns c and c+1 do not collide in the LDS banks
     const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
-    const int no = G.j1 - G.j0;
-    uint4 *bl = lh + LX * LYP;  // [no][64] weights
-    int4 *orec = reinterpret_cast<int4 *>(bl + no * 64);  // [no] (ox, oy, box delta, -)
+    const int no = G.j1 - G.j0, np = G.pad1;
+    uint4 *bl = lh + LX * LYP;  // [no + np][FSM_BMAT_U4] packed weights: offsets, then pair MFMAs
+    int4 *orec = reinterpret_cast<int4 *>(bl + (no + np) * FSM_BMAT_U4);  // offsets (ox, oy, box delta, -), pairs (delta gy=0, gy=1, -, -)
     __syncthreads();
     group_store(pf, ev, G, lh, x0, y0, TX, TY);
     __syncthreads();
@@ -507,7 +554,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     const bool interior = (xw0 + G.bx0 >= 1) && (xw1 + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
                           (ylast + G.by1 - 1 <= ny - 2);
     if (interior)
-      mfma_offsets<XW, false>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
+      mfma_pairs<XW>(acc, lh, bl + no * FSM_BMAT_U4, orec + no, np, lane_base - gy, LYP, gx, gy);
     else
       mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
   }
