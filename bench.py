@@ -299,9 +299,11 @@ def main():
     info = ctx.plan_info(q1)  # context is on the mixed path here
     samples = D * q1.DIM_PHIP * q1.DIM_RAP * B
     k_ms = mixed["stream_launch_ms"]
-    # MFMA form: per (cell, knot offset) one 16x16x32 product row, i.e.
-    # 16*32*2 = 1024 FLOP per cell and offset (12 of the 16 columns are used)
-    mfma_flops = B * D * info["n_offsets"] * 1024
+    # MFMA instructions the stream kernel executes per event (the plan mirrors
+    # the kernel's per-wave choice: the knot-displacement form where no mask
+    # applies, the per-offset form at the grid edges), 16*16*32*2 FLOP each
+    # (12 of the 16 columns carry moments)
+    mfma_flops = B * info["mfma_batch"] * 16384
     achieved = mfma_flops / (k_ms * 1e-3) / 1e12
     roofline = {
         "kernel": "k_stream2d_mfma_batch",
@@ -314,10 +316,15 @@ def main():
         "avg_kernel_ms": round(k_ms, 5),
         "flop_per_launch": mfma_flops,
         "fp32_equivalent_tflops": round(FLOP_PER_SAMPLE_2D * samples / (k_ms * 1e-3) / 1e12, 3),
-        "note": "v_mfma_f32_16x16x32_f16 work of the knot-offset form: %d events x %d cells x %d knot offsets "
-                "x 1024 FLOP per launch, against MI355X's dense FP16 peak; fp32_equivalent counts the "
-                "reference's 52 FLOP per (cell, phi) sample over the same time; traffic = HBM bytes per "
-                "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)" % (B, D, info["n_offsets"]),
+        "mfma_per_event": info["mfma_batch"],
+        # the same time against the per-offset form's work (one MFMA per 16 cells and knot offset), the
+        # measure this kernel was quoted in before the knot-displacement form cut the instruction count
+        "offset_form_equivalent_tflops": round(B * D * info["n_offsets"] * 1024 / (k_ms * 1e-3) / 1e12, 3),
+        "note": "v_mfma_f32_16x16x32_f16 instructions the kernel executes: %d events x %d per event x 16384 FLOP "
+                "per launch (per 16 cells: %.1f, against %d knot offsets of the per-offset form), against "
+                "MI355X's dense FP16 peak; fp32_equivalent counts the reference's 52 FLOP per (cell, phi) sample "
+                "over the same time; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                "(profiles/)" % (B, info["mfma_batch"], info["mfma_batch"] * 16.0 / D, info["n_offsets"]),
     }
     kf = fp32["stream_launch_ms"]
     roofline_fp32 = {

@@ -131,6 +131,8 @@ struct Plan {
   DevBuf<int4> moffs;
   DevBuf<uint4> bmat;
   DevBuf<StencilGroup> mgroups;
+  std::vector<StencilGroup> mgroups_h;  // host copies (MFMA instruction count)
+  std::vector<int4> moffs_h;
   DevBuf<int> amb;
   DevBuf<uint4> planes_h;
   DevBuf<unsigned> sp_flags;
@@ -548,6 +550,8 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   FSM_TRY(pl.moffs.upload(mt.recs));
   FSM_TRY(pl.bmat.upload(mt.bmat));
   FSM_TRY(pl.mgroups.upload(mt.groups));
+  pl.mgroups_h = mt.groups;
+  pl.moffs_h = mt.offs;
   FSM_TRY(pl.amb.upload(amb));
   const int stx = (nx + FSMSP_TX - 1) / FSMSP_TX, sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
   FSM_TRY(pl.planes_h.ensure((size_t)nx * ny));
@@ -1102,6 +1106,44 @@ int fsm_set_path(fsm_ctx *ctx, int path) {
   return FSM_OK;
 }
 
+namespace {
+// MFMA instructions of one event's MFMA stream stage on tile shape `variant`,
+// following the kernel's decisions (fsm_stream2d.hip, stream2d_mfma_tile): a
+// wave whose cells keep every sample of a group runs the group's pair MFMAs
+// for each of its XW columns and 2 row blocks; otherwise each offset's MFMAs
+// for the row blocks and columns whose floor indices can be in range.  Tiles
+// that fall back to the FP32 loop for spline undershoot are not subtracted.
+long mfma_count(const Plan &pl, int variant, int nx, int ny) {
+  const int TX = fsm_stream2d_tile_w(variant), TY = fsm_stream2d_tile_h(variant), XW = TX / 4;
+  long n = 0;
+  for (int x0 = 0; x0 < nx; x0 += TX)
+    for (int y0 = 0; y0 < ny; y0 += TY) {
+      const int ylast = std::min(y0 + TY - 1, ny - 1);
+      for (int w = 0; w < 4; ++w) {
+        const int xw0 = x0 + w * XW, xw1 = std::min(xw0 + XW - 1, nx - 1);
+        for (const StencilGroup &G : pl.mgroups_h) {
+          const bool interior = (xw0 + G.bx0 >= 1) && (xw1 + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
+                                (ylast + G.by1 - 1 <= ny - 2);
+          if (interior) {
+            n += (long)G.pad1 * XW * 2;
+            continue;
+          }
+          for (int o = G.j0; o < G.j1; ++o) {
+            const int ox = pl.moffs_h[o].x, oy = pl.moffs_h[o].y;
+            for (int yb = 0; yb < 2; ++yb) {
+              const int ylo = y0 + yb * 16 + oy;
+              if (ylo + 15 < 1 || ylo > ny - 2) continue;
+              for (int xc = 0; xc < XW; ++xc)
+                if ((unsigned)(xw0 + xc + ox - 1) <= (unsigned)(nx - 3)) ++n;
+            }
+          }
+        }
+      }
+    }
+  return n;
+}
+}  // namespace
+
 int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info) {
   if (!ctx || !p || !info) return FSM_ERR_ARG;
   FSM_TRY(hipSetDevice(ctx->device));
@@ -1133,6 +1175,8 @@ int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info) {
                       ((p->DIM_Y + fsm_stream2d_tile_h(bv) - 1) / fsm_stream2d_tile_h(bv));
   info->tuned = pl->tuned[ctx->path] ? 1 : 0;
   for (int v = 0; v < 3; ++v) info->tune_us[v] = pl->tune_ms[ctx->path][v] * 1000.0f;
+  info->mfma_batch = (int)mfma_count(*pl, bv, p->DIM_X, p->DIM_Y);
+  info->mfma_one = (int)mfma_count(*pl, ev.variant, p->DIM_X, p->DIM_Y);
   return FSM_OK;
 }
 

@@ -101,7 +101,8 @@ class PlanInfo(ctypes.Structure):
 
     _fields_ = [(n, ctypes.c_int) for n in ("dim3", "variant", "tile_w", "tile_h", "tiles", "n_phi", "n_offsets",
                                             "n_offset_groups", "n_vector_samples", "lds_bytes", "batch_variant",
-                                            "batch_tiles", "tuned")] + [("tune_us", ctypes.c_float * 3)]
+                                            "batch_tiles", "tuned")] + [("tune_us", ctypes.c_float * 3)] + \
+        [(n, ctypes.c_int) for n in ("mfma_batch", "mfma_one")]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

@@ -149,6 +149,8 @@ typedef struct fsm_plan_info {
   int batch_tiles;      /* tiles of one event inside batches */
   int tuned;            /* 1 if batch_variant was chosen by the autotuner */
   float tune_us[3];     /* autotuner: stream time per event of each shape */
+  int mfma_batch;       /* MFMA instructions (16x16x32) of one event's stream stage on the batch shape */
+  int mfma_one;         /* the same on the one-event shape (both count every tile on the MFMA form) */
 } fsm_plan_info;
 int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info);
 
