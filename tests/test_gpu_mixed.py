@@ -55,6 +55,10 @@ def _case(name):
                           TAU0=0.1, DTAU=2.0, SIGMA=1.0)
         e2 = W.hotspots_2d(151, 101, 0.1, 0.1, 40, 17, width=0.6)
         return p, (e2[:, :, None] * np.ones((1, 1, 7))).astype(np.float32).ravel()
+    if name == "trento_ref":  # the reference's own example TRENTO event (tests/golden/trento_e.dat)
+        from test_gpu_parity import reference_trento_event
+
+        return reference_trento_event()
     if name == "int2d":  # DTAU/DX = 10: samples on grid lines next to the mask edge
         p = W.base_params(DIM_X=101, DIM_Y=101, DIM_PHIP=64, DX=0.08, DY=0.08, TAU0=0.1, DTAU=0.8)
         return p, W.hotspots_2d(101, 101, 0.08, 0.08, 60, 13).ravel()
@@ -75,8 +79,8 @@ def _gpu(ctx, q, e):
     return d_T.cpu().numpy().reshape(10, D), d_out.cpu().numpy().reshape(17, D)
 
 
-@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d", "small3d_int",
-                                  "rect3d", "wide3d"])
+@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "trento_ref", "small3d",
+                                  "small3d_int", "rect3d", "wide3d"])
 def test_mixed_parity(mixed_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)

@@ -30,7 +30,20 @@ def _case(name):
         from test_gpu_mixed import _case as mixed_case
 
         return mixed_case(name)
+    if name == "trento_ref":
+        return reference_trento_event()
     return W.config(name)
+
+
+def reference_trento_event():
+    """The reference's own example profile (initial_profiles/e.dat, a 131^2
+    TRENTO event, committed as tests/golden/trento_e.dat) through the TRENTO
+    block reader, streamed on a 0.1 fm grid for 1 fm/c.  The reader gives the
+    density in fm^-4, as for every IC_ENERGY but the vector input (here 3)."""
+    p = W.base_params(DIM_X=131, DIM_Y=131, DX=0.1, DY=0.1, DIM_PHIP=64, TAU0=0.1, DTAU=1.0)
+    p.IC_ENERGY = 3
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trento_e.dat")
+    return p, fsm_amd.read_profile(p, fsm_amd.PROFILE_TRENTO_BLOCK, path)
 
 
 def _gpu(ctx, q, e):
@@ -47,7 +60,8 @@ def _gpu(ctx, q, e):
     return d_T.cpu().numpy().reshape(10, D), d_out.cpu().numpy().reshape(17, D)
 
 
-@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d", "small3d_int"])
+@pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d", "small3d_int",
+                                  "trento_ref"])
 def test_parity_against_reference_float_path(gpu_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)

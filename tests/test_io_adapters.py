@@ -168,6 +168,30 @@ def test_block_readers(tmp_path):
     assert np.array_equal(got, v3 + np.float32(1e-3))
 
 
+def test_reference_profile_files():
+    """The reference's own example profiles (initial_profiles/e.dat, a 131^2
+    TRENTO event with a 12-line header, rows along y; initial_profiles/
+    ed_event_1_block.dat, a 51^2 superMC block, rows along x), committed as
+    tests/golden/trento_e.dat and supermc_ed_event_1_block.dat, read exactly:
+    value + 1e-3f on a 2-D grid (the eta envelope is 1 at eta = 0)."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    p = _params()
+    p.DIM_X = p.DIM_Y = 131
+    p.DIM_ETA = 1
+    got = fsm_amd.read_profile(p, fsm_amd.PROFILE_TRENTO_BLOCK, os.path.join(gold, "trento_e.dat"))
+    data = np.loadtxt(os.path.join(gold, "trento_e.dat"), comments="#", dtype=np.float64)
+    assert data.shape == (131, 131)
+    tol = np.float64(np.float32(1e-3))  # the reference's float lower_tolerance, added in double
+    want = (data.T.astype(np.float32).astype(np.float64) + tol).astype(np.float32)  # [ix, iy]
+    assert np.array_equal(got, want.ravel())
+    p.DIM_X = p.DIM_Y = 51
+    got = fsm_amd.read_profile(p, fsm_amd.PROFILE_SUPERMC_BLOCK, os.path.join(gold, "supermc_ed_event_1_block.dat"))
+    data = np.loadtxt(os.path.join(gold, "supermc_ed_event_1_block.dat"), dtype=np.float64)
+    assert data.shape == (51, 51)
+    want = (data.astype(np.float32).astype(np.float64) + tol).astype(np.float32)
+    assert np.array_equal(got, want.ravel())
+
+
 def _snapped(n, d):
     i = np.arange(n, dtype=np.float32)
     x = ((i * np.float32(d)).astype(np.float64) - (np.float64(np.float32(n - 1)) / 2.0 * np.float64(np.float32(d))))
