@@ -50,6 +50,43 @@ __device__ __forceinline__ int xcd_order(int b, int n) {
   return x * q + (x < rem ? x : rem) + j;
 }
 
+// Memory-space views of pointers held in the event descriptors.  A pointer
+// loaded from memory is generic, and the compiler emits flat loads through
+// it; a flat load counts in both the vector-memory and the LDS/scalar wait
+// counters, so any wait for an LDS read also waits for every flat load in
+// flight, and prefetches issued ahead of an LDS-heavy loop are waited for at
+// its first LDS read.  Global loads count only in the vector-memory counter.
+// Scalar loads (constant address space) are for wave-uniform addresses of
+// data no kernel writes during the launch (plan tables, descriptors).
+#define FSM_AS_GLOBAL __attribute__((address_space(1)))
+#define FSM_AS_CONST __attribute__((address_space(4)))
+typedef unsigned fsm_v4u __attribute__((ext_vector_type(4)));
+typedef unsigned fsm_v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 fsm_gload(const uint4 *p) {
+  const fsm_v4u v = *(const FSM_AS_GLOBAL fsm_v4u *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 fsm_gload(const float4 *p) {
+  const fsm_v4u v = *(const FSM_AS_GLOBAL fsm_v4u *)p;
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+__device__ __forceinline__ uint2 fsm_gload(const uint2 *p) {
+  const fsm_v2u v = *(const FSM_AS_GLOBAL fsm_v2u *)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ unsigned fsm_gload(const unsigned *p) { return *(const FSM_AS_GLOBAL unsigned *)p; }
+__device__ __forceinline__ void fsm_gstore(float *p, float v) { *(FSM_AS_GLOBAL float *)p = v; }
+// Scalar load of a trivially copyable descriptor (size a multiple of 4 bytes).
+template <typename T>
+__device__ __forceinline__ T fsm_sload(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "scalar loads move whole dwords");
+  unsigned w[sizeof(T) / 4];
+  const FSM_AS_CONST unsigned *s = (const FSM_AS_CONST unsigned *)p;
+#pragma unroll
+  for (unsigned k = 0; k < sizeof(T) / 4; ++k) w[k] = s[k];
+  return __builtin_bit_cast(T, w);
+}
+
 // ---------------------------------------------------------------- 2-D ----
 // Cells per lane along x in the 2-D stream kernel; one wave covers 64
 // consecutive iy.  Tile = (4 waves * FSM2D_M) x 64 cells.

@@ -733,16 +733,12 @@ __device__ __forceinline__ double jrsq(double x) {
 // gamma recomputed so that it is a Lorentz transformation to working
 // precision, and M starts at B(w).  The sweeps then only remove what is left
 // of A^{0i}; the frame they converge to is the same.
-// N = 4: the general problem.  N = 3: a 2-D event's tensor, whose eta
-// components T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} are zero (the
-// hypertrig rows carry tanh(0) = 0, src/LandauMatch.cpp:37-55); the eta row
-// and column of A and M then never change, so dropping them gives the same
-// frame with fewer operations and registers (t[3], t[6], t[8], t[9] are not
-// read; u^eta = 0).
-template <typename R, int N = 4>
+// (2-D events, whose tensors from the stream kernel have zero eta
+// components, take the closed 3x3 form below instead: landau_cubic.)
+template <typename R>
 __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4],
                              const R *w = nullptr) {
-  static_assert(N == 3 || N == 4, "Landau frame in 2+1 or 3+1 dimensions");
+  constexpr int N = 4;
   const R tau = (R)TAU;
   R A[N][N], M[N][N];
   A[0][0] = t[0];
@@ -751,18 +747,16 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
   A[1][1] = t[4];
   A[1][2] = A[2][1] = t[5];
   A[2][2] = t[7];
-  if constexpr (N == 4) {
-    A[0][3] = A[3][0] = tau * (R)t[3];
-    A[1][3] = A[3][1] = tau * (R)t[6];
-    A[2][3] = A[3][2] = tau * (R)t[8];
-    A[3][3] = tau * tau * (R)t[9];
-  }
+  A[0][3] = A[3][0] = tau * (R)t[3];
+  A[1][3] = A[3][1] = tau * (R)t[6];
+  A[2][3] = A[3][2] = tau * (R)t[8];
+  A[3][3] = tau * tau * (R)t[9];
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) M[i][j] = (i == j) ? (R)1 : (R)0;
   if (w) {
-    const R v[4] = {(R)0, w[0], w[1], N == 4 ? w[2] : (R)0};
+    const R v[4] = {(R)0, w[0], w[1], w[2]};
     const R g = sqrt((R)1 + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
     const R h = (R)1 / ((R)1 + g);
     // B(w) = [[g, w^T], [w, I + w w^T / (1 + g)]]; B(-w) flips the 0i signs
@@ -792,9 +786,7 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
       }
   }
   auto off_diag = [&]() {
-    R o = A[0][1] * A[0][1] + A[0][2] * A[0][2];
-    if constexpr (N == 4) o += A[0][3] * A[0][3];
-    return o;
+    return A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
   };
   bool ok = true, conv = false;
   for (int sweep = 0; sweep < 60 && ok; ++sweep) {
@@ -851,10 +843,7 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
     u_out[0] = (float)u0;
     u_out[1] = (float)M[1][0];
     u_out[2] = (float)M[2][0];
-    if constexpr (N == 4)
-      u_out[3] = (float)(M[N - 1][0] / tau);
-    else
-      u_out[3] = 0.0f;
+    u_out[3] = (float)(M[3][0] / tau);
     return 1;
   }
   *e_out = 1.0e-3f;
@@ -867,9 +856,8 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
 // (on sampled cells of a lumpy 2-D and a 3-D event its float results equal
 // those of a 1e-14 criterion; the eigenvalue error is quadratic in the
 // residual).
-template <int N>
 __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4], const double *w = nullptr) {
-  (void)landau_jacobi<double, N>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out, w);
+  (void)landau_jacobi<double>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out, w);
 }
 
 // FP32 stage of the mixed path.  Returns true when FP32 decides the result to
@@ -878,9 +866,8 @@ __device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_
 // small difference of large terms (kappa > 16: nearly null tensors in dilute
 // edge cells, where FP32 rounding in T is amplified by kappa).  Those cells
 // are solved again in FP64.
-template <int N>
 __device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float u_out[4], int *st_out) {
-  const int st = landau_jacobi<float, N>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
+  const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
   *st_out = st;
   // First-order sensitivity of e = T_{mu nu} u^mu u^nu to a relative change
   // of every component: kappa = sum |u_mu| |T^{mu nu}| |u_nu| / e (>= u0^2 T^tt / e).
@@ -889,6 +876,75 @@ __device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float
                   2.0f * (a0 * (a1 * fabsf(t[1]) + a2 * fabsf(t[2]) + a3 * fabsf(t[3])) +
                           a1 * (a2 * fabsf(t[5]) + a3 * fabsf(t[6])) + a2 * a3 * fabsf(t[8]));
   return st == 1 && u_out[0] <= 1.5f && S <= 16.0f * *e_out;
+}
+
+// Landau frame of a 2-D event's tensor (eta components zero), FP64.  With
+// A = T eta (A^mu_nu = T^{mu alpha} eta_{alpha nu}, 3x3), e is an eigenvalue
+// of A with a timelike eigenvector, and for a tensor of free-streamed
+// massless particles (a positive sum of null dyads) it is the largest root of
+// the characteristic cubic det(A - lambda) = 0: the others are -P_1, -P_2 with
+// pressures P_i >= 0.  e is also the minimum of T(n, n) over unit timelike n,
+// so T(n, n) at n = gamma (1, v), v = T^{0i} / (T^{00} + (T^{xx} + T^{yy})/3),
+// lies above every root; Halley's iteration from there converges to e (five
+// or six steps on the lumpy and noisy test events, from the first; cubic
+// convergence near the root).  u is the null vector of A - e: the largest of
+// the three cross products of its rows, normalised to u.u = 1 with u^tau > 0.
+// The reference's rules (src/LandauMatch.cpp:222-291) then decide: e > 0,
+// u timelike, u^tau < 100, else e = 1e-3, u = (1, 0, 0, 0).  Returns 1 when
+// the frame is found.
+__device__ __forceinline__ double lrcp(double x) {
+  // hardware reciprocal estimate refined by two Newton steps
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+__device__ int landau_cubic(const float t[10], float *e_out, float u_out[4]) {
+  const double t0 = t[0], t1 = t[1], t2 = t[2], t4 = t[4], t5 = t[5], t7 = t[7];
+  // A = T eta: rows (t0, -t1, -t2), (t1, -t4, -t5), (t2, -t5, -t7)
+  const double a00 = t0, a01 = -t1, a02 = -t2, a10 = t1, a11 = -t4, a12 = -t5, a20 = t2, a21 = -t5, a22 = -t7;
+  const double tr = a00 + a11 + a22;
+  const double m12 = a11 * a22 - a12 * a21;
+  const double c2 = (a00 * a11 - a01 * a10) + (a00 * a22 - a02 * a20) + m12;
+  const double det = a00 * m12 - a01 * (a10 * a22 - a12 * a20) + a02 * (a10 * a21 - a11 * a20);
+  const double rd = lrcp(t0 + (t4 + t7) * (1.0 / 3.0));
+  const double vx = t1 * rd, vy = t2 * rd;
+  const double g2 = lrcp(1.0 - vx * vx - vy * vy);
+  const double tnn = g2 * (t0 - 2.0 * (vx * t1 + vy * t2) + vx * vx * t4 + 2.0 * vx * vy * t5 + vy * vy * t7);
+  double lam = fmin(t0, tnn);  // fmin: T^{00} where the estimate is not a number
+  for (int it = 0; it < 12; ++it) {
+    const double p = ((lam - tr) * lam + c2) * lam - det;
+    const double dp = (3.0 * lam - 2.0 * tr) * lam + c2;
+    const double d2 = 6.0 * lam - 2.0 * tr;
+    const double st = 2.0 * p * dp * lrcp(2.0 * dp * dp - p * d2);
+    lam -= st;
+    if (!(fabs(st) > 1.0e-13 * fabs(lam))) break;  // converged (the next step is below rounding), or not a number
+  }
+  // rows of A - e and their cross products
+  const double b00 = a00 - lam, b11 = a11 - lam, b22 = a22 - lam;
+  const double p0 = a01 * a12 - a02 * b11, p1 = a02 * a10 - b00 * a12, p2 = b00 * b11 - a01 * a10;  // r0 x r1
+  const double q0 = a01 * b22 - a02 * a21, q1 = a02 * a20 - b00 * b22, q2 = b00 * a21 - a01 * a20;  // r0 x r2
+  const double s0 = b11 * b22 - a12 * a21, s1 = a12 * a20 - a10 * b22, s2 = a10 * a21 - b11 * a20;  // r1 x r2
+  const double np = p0 * p0 + p1 * p1 + p2 * p2, nq = q0 * q0 + q1 * q1 + q2 * q2, ns = s0 * s0 + s1 * s1 + s2 * s2;
+  const bool use_p = np >= nq && np >= ns, use_q = !use_p && nq >= ns;
+  const double v0 = use_p ? p0 : use_q ? q0 : s0, v1 = use_p ? p1 : use_q ? q1 : s1, v2 = use_p ? p2 : use_q ? q2 : s2;
+  const double ml = v0 * v0 - (v1 * v1 + v2 * v2);
+  if (lam > 0.0 && ml > 0.0 && isfinite(lam) && isfinite(ml)) {
+    double f = jrsq(ml);
+    if (v0 < 0.0) f = -f;
+    const double u0 = v0 * f;
+    if (u0 < 100.0) {
+      *e_out = (float)lam;
+      u_out[0] = (float)u0;
+      u_out[1] = (float)(v1 * f);
+      u_out[2] = (float)(v2 * f);
+      u_out[3] = 0.0f;
+      return 1;
+    }
+  }
+  *e_out = 1.0e-3f;
+  u_out[0] = 1.0f;
+  u_out[1] = u_out[2] = u_out[3] = 0.0f;
+  return 0;
 }
 
 __device__ float eos_pressure(float e, int eos) {
@@ -945,8 +1001,9 @@ struct LandauShared {
 };
 
 // Landau frame of the workgroup's cells (one per thread, `live` marks real
-// cells; every thread of the workgroup calls this).  FP32 path: FP64 Jacobi for
-// every cell.  Mixed path: FP32 Jacobi for every cell, then the cells that
+// cells; every thread of the workgroup calls this).  N = 3 (2-D events): the
+// FP64 cubic per cell on both paths.  N = 4: FP32 path, FP64 Jacobi for
+// every cell; mixed path, FP32 Jacobi for every cell, then the cells that
 // FP32 cannot decide are gathered into a dense list and re-solved in FP64 by
 // the first threads of the workgroup, so FP64 work is not spread over
 // partly-idle waves; a converged FP32 frame is the FP64 sweeps' starting
@@ -954,44 +1011,52 @@ struct LandauShared {
 template <int N>
 __device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, const float t[10], float *e,
                                    float u[4], LandauShared &S) {
-  if (!ev.mixed) {
-    if (live) landau_fp64<N>(t, ev.tau, e, u);
-    return;
-  }
-  bool need = false;
-  int st = 0;
-  if (live) need = !landau_fp32_ok<N>(t, ev.tau, e, u, &st);
-  __syncthreads();  // the previous group's list is no longer read
-  if (threadIdx.x == 0) S.n64 = 0;
-  __syncthreads();
-  if (need) {
-    const int k = atomicAdd(&S.n64, 1);
-    S.who[k] = threadIdx.x;
-    S.tidx[threadIdx.x] = (int)tidx;
-    S.warm[threadIdx.x] = st;
+  if constexpr (N == 3) {
+    // 2-D events, both paths: the FP64 cubic (no workgroup list needed)
+    (void)ev;
+    (void)tidx;
+    (void)S;
+    if (live) landau_cubic(t, e, u);
+  } else {
+    if (!ev.mixed) {
+      if (live) landau_fp64(t, ev.tau, e, u);
+      return;
+    }
+    bool need = false;
+    int st = 0;
+    if (live) need = !landau_fp32_ok(t, ev.tau, e, u, &st);
+    __syncthreads();  // the previous group's list is no longer read
+    if (threadIdx.x == 0) S.n64 = 0;
+    __syncthreads();
+    if (need) {
+      const int k = atomicAdd(&S.n64, 1);
+      S.who[k] = threadIdx.x;
+      S.tidx[threadIdx.x] = (int)tidx;
+      S.warm[threadIdx.x] = st;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) S.u[c][threadIdx.x] = u[c];
-  }
-  __syncthreads();
-  const int n = S.n64;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int w = S.who[k];
-    const long ti = S.tidx[w];
-    float tt[10], e64, u64[4];
+      for (int c = 0; c < 4; ++c) S.u[c][threadIdx.x] = u[c];
+    }
+    __syncthreads();
+    const int n = S.n64;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int w = S.who[k];
+      const long ti = S.tidx[w];
+      float tt[10], e64, u64[4];
 #pragma unroll
-    for (int c = 0; c < 10; ++c) tt[c] = ev.T[c * ev.t_plane + ti];
-    // the FP32 frame (orthonormal components) seeds the FP64 sweeps
-    const double w3[3] = {(double)S.u[1][w], (double)S.u[2][w], (double)ev.tau * (double)S.u[3][w]};
-    landau_fp64<N>(tt, ev.tau, &e64, u64, S.warm[w] ? w3 : nullptr);
-    S.e[w] = e64;
+      for (int c = 0; c < 10; ++c) tt[c] = ev.T[c * ev.t_plane + ti];
+      // the FP32 frame (orthonormal components) seeds the FP64 sweeps
+      const double w3[3] = {(double)S.u[1][w], (double)S.u[2][w], (double)ev.tau * (double)S.u[3][w]};
+      landau_fp64(tt, ev.tau, &e64, u64, S.warm[w] ? w3 : nullptr);
+      S.e[w] = e64;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) S.u[c][w] = u64[c];
-  }
-  __syncthreads();
-  if (need) {
-    *e = S.e[threadIdx.x];
+      for (int c = 0; c < 4; ++c) S.u[c][w] = u64[c];
+    }
+    __syncthreads();
+    if (need) {
+      *e = S.e[threadIdx.x];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) u[c] = S.u[c][threadIdx.x];
+      for (int c = 0; c < 4; ++c) u[c] = S.u[c][threadIdx.x];
+    }
   }
 }
 

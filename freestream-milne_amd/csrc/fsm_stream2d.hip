@@ -250,7 +250,7 @@ typedef _Float16 fsm_half8 __attribute__((ext_vector_type(8)));
 typedef float fsm_float4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float4 plane_or_zero(const Event2D &ev, int gx, int gy) {
-  if ((unsigned)gx < (unsigned)ev.nx && (unsigned)gy < (unsigned)ev.ny) return ev.planes[(long)gx * ev.ny + gy];
+  if ((unsigned)gx < (unsigned)ev.nx && (unsigned)gy < (unsigned)ev.ny) return fsm_gload(ev.planes + (long)gx * ev.ny + gy);
   return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -443,29 +443,38 @@ struct BoxWalk {
   }
 };
 
-__device__ __forceinline__ void group_load(GroupPrefetch &pf, const Event2D &ev, const StencilGroup &G, int x0,
+__device__ __forceinline__ void group_load(GroupPrefetch &pf, const Event2D &ev, const StencilGroup *gp, int x0,
                                            int y0, int TX, int TY) {
   const int nx = ev.nx, ny = ev.ny;
-  const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0;
-  const int n = LX * LY, bx = x0 + G.bx0, by = y0 + G.by0;
+  // The descriptor's fields are needed before any load of the group is
+  // issued: the empty asm makes them available here, so that the wait for
+  // them cannot land after the box loads (and wait for those too).
+  int bx = x0 + gp->bx0, by = y0 + gp->by0, LX = TX + gp->bx1 - gp->bx0, LY = TY + gp->by1 - gp->by0;
+  int nm = gp->j1 - gp->j0 + gp->pad1, p0 = gp->pad0;
+  asm volatile("" : "+v"(bx), "+v"(by), "+v"(LX), "+v"(LY), "+v"(nm), "+v"(p0));
+  const int n = LX * LY;
   BoxWalk wk(LY);
 #pragma unroll
   for (int u = 0; u < kPfBox; ++u) {
     const int k = threadIdx.x + u * blockDim.x;
     const int kx = bx + wk.lx, ky = by + wk.ly;
     pf.box[u] = make_uint4(0u, 0u, 0u, 0u);
-    if (k < n && (unsigned)kx < (unsigned)nx && (unsigned)ky < (unsigned)ny) pf.box[u] = ev.planes_h[(long)kx * ny + ky];
+    if (k < n && (unsigned)kx < (unsigned)nx && (unsigned)ky < (unsigned)ny)
+      pf.box[u] = fsm_gload(ev.planes_h + (long)kx * ny + ky);
     wk.next();
   }
   // the group's block (G.pad0): its offsets' records and B operands, then
   // its G.pad1 pair MFMAs'
-  const int nm = G.j1 - G.j0 + G.pad1, nw = nm * FSM_BMAT_U4;
+  const int nw = nm * FSM_BMAT_U4;
 #pragma unroll
   for (int u = 0; u < kPfW; ++u) {
     const int k = threadIdx.x + u * blockDim.x;
-    if (k < nw) pf.w[u] = ev.bmat[(long)G.pad0 * FSM_BMAT_U4 + k];
+    if (k < nw) pf.w[u] = fsm_gload(ev.bmat + (long)p0 * FSM_BMAT_U4 + k);
   }
-  if ((int)threadIdx.x < nm) pf.rec = ev.moffs[G.pad0 + threadIdx.x];
+  if ((int)threadIdx.x < nm) {
+    const uint4 r = fsm_gload(reinterpret_cast<const uint4 *>(ev.moffs + p0 + threadIdx.x));
+    pf.rec = make_int4((int)r.x, (int)r.y, (int)r.z, (int)r.w);
+  }
 }
 
 __device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event2D &ev, const StencilGroup &G,
@@ -484,7 +493,7 @@ __device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event
     const int lx = k / LY, ly = k - lx * LY;
     const int kx = x0 + G.bx0 + lx, ky = y0 + G.by0 + ly;
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if ((unsigned)kx < (unsigned)ev.nx && (unsigned)ky < (unsigned)ev.ny) v = ev.planes_h[(long)kx * ev.ny + ky];
+    if ((unsigned)kx < (unsigned)ev.nx && (unsigned)ky < (unsigned)ev.ny) v = fsm_gload(ev.planes_h + (long)kx * ev.ny + ky);
     lh[lx * LYP + ly] = v;
   }
   const int nw = no * FSM_BMAT_U4;
@@ -493,7 +502,16 @@ __device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event
     const int k = threadIdx.x + u * step;
     if (k < nw) bl[k] = pf.w[u];
   }
-  for (int k = threadIdx.x + kPfW * step; k < nw; k += step) bl[k] = ev.bmat[(long)G.pad0 * FSM_BMAT_U4 + k];
+  const uint4 *bsrc = ev.bmat + (long)G.pad0 * FSM_BMAT_U4;
+  for (int k0 = threadIdx.x + kPfW * step; k0 < nw; k0 += 4 * step) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u * step < nw) v[u] = fsm_gload(bsrc + k0 + u * step);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u * step < nw) bl[k0 + u * step] = v[u];
+  }
   if ((int)threadIdx.x < no) orec[threadIdx.x] = pf.rec;
 }
 
@@ -517,7 +535,8 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     const int sy0 = max(0, y0 + ev.rby0) / FSMSP_TY, sy1 = min(ny - 1, y0 + TY - 1 + ev.rby1) / FSMSP_TY;
     const int cy = sy1 - sy0 + 1, cnt = (sx1 - sx0 + 1) * cy;
     int bad = 0;
-    for (int k = tid; k < cnt; k += blockDim.x) bad |= ev.sp_flags[(sx0 + k / cy) * ev.sp_tiles_y + sy0 + k % cy] != 0u;
+    for (int k = tid; k < cnt; k += blockDim.x)
+      bad |= fsm_gload(ev.sp_flags + (sx0 + k / cy) * ev.sp_tiles_y + sy0 + k % cy) != 0u;
     if (__syncthreads_or(bad)) {
       stream2d_tile_v2<MfmaShape<V>::M, MfmaShape<V>::JS>(ev, tile_x, tile_y, lds);
       return;
@@ -532,7 +551,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     for (int yb = 0; yb < 2; ++yb) acc[xc][yb] = fsm_float4v{0.f, 0.f, 0.f, 0.f};
   const int ylast = min(y0 + TY - 1, ny - 1);
   GroupPrefetch pf;
-  if (ev.n_mgroups > 0) group_load(pf, ev, ev.mgroups[0], x0, y0, TX, TY);
+  if (ev.n_mgroups > 0) group_load(pf, ev, ev.mgroups, x0, y0, TX, TY);
   for (int gi = 0; gi < ev.n_mgroups; ++gi) {
     const StencilGroup G = ev.mgroups[gi];
     // box columns are padded to 16 records (256 B) so that the lanes reading
@@ -545,7 +564,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     group_store(pf, ev, G, lh, x0, y0, TX, TY);
     __syncthreads();
     // next group's loads fly while this group's MFMAs run
-    if (gi + 1 < ev.n_mgroups) group_load(pf, ev, ev.mgroups[gi + 1], x0, y0, TX, TY);
+    if (gi + 1 < ev.n_mgroups) group_load(pf, ev, ev.mgroups + gi + 1, x0, y0, TX, TY);
     // every real cell's floor index stays inside [1, n-2] for all offsets of the group
     // lane's knot (corner) for cell column wave*XW, row r, at box offset 0
     const int lane_base = (wave * XW + gx) * LYP + r + gy;
@@ -577,7 +596,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
   __syncthreads();
   // src/LandauMatch.cpp:104: T = sum * d_phip / TAU; rows 3, 6, 8, 9 carry tanh(0) = 0
   const long P = ev.t_plane;
-  const float inv_sc = 1.0f / ev.mscale[FSM_KS_PARTS];  // a power of two: exact
+  const float inv_sc = 1.0f / fsm_sload(ev.mscale + FSM_KS_PARTS);  // a power of two: exact
   constexpr int CPT = (NC + 255) / 256;     // cells per thread
   // cell q of this thread: c = tid + 256 q, column c / TY, row c % TY (TY =
   // 32: shifts, recomputed where used rather than held in registers)
@@ -615,24 +634,24 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
       // src/LandauMatch.cpp:140: J = sum * DRAP * d_phip; rows 1, c, s, tanh(0) = 0
       const long is = (long)cell_x(q) * ny + cell_y(q);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ev.T[c * P + is] = ((t[q][c] * ev.out_scale) * ev.jscale) * ev.d_phip;
-      ev.T[3 * P + is] = 0.0f;
+      for (int c = 0; c < 3; ++c) fsm_gstore(ev.T + c * P + is, ((t[q][c] * ev.out_scale) * ev.jscale) * ev.d_phip);
+      fsm_gstore(ev.T + 3 * P + is, 0.0f);
       continue;
     }
     float rr[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) rr[m] = ((t[q][m] * ev.out_scale) * ev.d_phip) / ev.tau;
     const long is = (long)cell_x(q) * ny + cell_y(q);
-    ev.T[0 * P + is] = rr[0];
-    ev.T[1 * P + is] = rr[1];
-    ev.T[2 * P + is] = rr[2];
-    ev.T[3 * P + is] = 0.0f;
-    ev.T[4 * P + is] = rr[3];
-    ev.T[5 * P + is] = rr[4];
-    ev.T[6 * P + is] = 0.0f;
-    ev.T[7 * P + is] = rr[5];
-    ev.T[8 * P + is] = 0.0f;
-    ev.T[9 * P + is] = 0.0f;
+    fsm_gstore(ev.T + 0 * P + is, rr[0]);
+    fsm_gstore(ev.T + 1 * P + is, rr[1]);
+    fsm_gstore(ev.T + 2 * P + is, rr[2]);
+    fsm_gstore(ev.T + 3 * P + is, 0.0f);
+    fsm_gstore(ev.T + 4 * P + is, rr[3]);
+    fsm_gstore(ev.T + 5 * P + is, rr[4]);
+    fsm_gstore(ev.T + 6 * P + is, 0.0f);
+    fsm_gstore(ev.T + 7 * P + is, rr[5]);
+    fsm_gstore(ev.T + 8 * P + is, 0.0f);
+    fsm_gstore(ev.T + 9 * P + is, 0.0f);
   }
 }
 

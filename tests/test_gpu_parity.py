@@ -197,3 +197,32 @@ def test_landau_stage_on_identical_input(gpu_ctx, name):
     fb = (eo == 1e-3) != (eg == np.float32(1e-3))
     print(name, "stage B: e rel err %.3g, u rel err %.3g, fallback mismatches %d" % (ee, ue, fb.sum()), flush=True)
     assert ee < 1e-5 and ue < 1e-5
+
+
+@pytest.mark.parametrize("path", ["fp32", "mixed"])
+@pytest.mark.parametrize("name", ["C1", "hotspots_spikes", "spiky", "int2d"])
+def test_landau_2d_frame_on_gpu_tensor(name, path):
+    """Stage B as the run path does it for 2-D events (the 3x3 frame from the
+    characteristic cubic, both paths): e and u of every cell with e > 1e-3
+    e_max against the oracle's FP64 eigen-solve of the GPU's own T^{mu nu}
+    (no conditioning exclusion), and the reference's fallback decision on
+    every cell."""
+    p, e = _case(name)
+    q = fsm_amd.params_finalize(p, e)
+    ctx = fsm_amd.Context(0)
+    ctx.set_path(fsm_amd.PATH_MIXED if path == "mixed" else fsm_amd.PATH_FP32)
+    T, out = _gpu(ctx, q, e)
+    ctx.close()
+    hb = np.float64(np.float32(0.197326938))
+    eg = out[0] / hb  # (double)e * hbarc is exact, so this is the float e
+    ug = out[2:6]
+    eo, uo = O.landau(T.astype(np.float64).T, np.float32(q.TAU))
+    m = eo > 1e-3 * eo.max()
+    ee = (np.abs(eg - eo)[m] / eo[m]).max()
+    ue = (np.abs(ug.T - uo)[m].max(axis=1) / uo[m, 0]).max()
+    fo = (eo == 1e-3) & (uo[:, 0] == 1.0)
+    fg = (eg == np.float32(1e-3)) & (ug[0] == 1.0)
+    print(name, path, "2-D stage B: e rel err %.3g, u rel err %.3g, fallbacks %d, mismatches %d" %
+          (ee, ue, int(fo.sum()), int((fo != fg).sum())), flush=True)
+    assert ee < 1e-6 and ue < 1e-6
+    assert np.array_equal(fo, fg)
