@@ -1006,6 +1006,10 @@ EventLandau landau_event(const fsm_params &p, const float *T, int eta_major, int
   L.fill_outputs = (p.OUTPUTFORMAT == 2 || p.OUTPUTFORMAT == 3) ? 1 : 0;
   L.mixed = mixed;
   L.tau = p.TAU;
+  {
+    const float t2 = p.TAU * p.TAU;  // a float product, as the reference forms it
+    L.inv_tau2 = 1.0 / (double)t2;
+  }
   L.T = T;
   L.t_plane = p.DIM;
   L.out_plane = p.DIM;

@@ -283,6 +283,7 @@ struct EventLandau {
   int eta_zero;         // 1: T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} are zero by construction (a 2-D
                         // event's tensor from the stream kernel): the 3x3 Landau frame
   float tau;
+  double inv_tau2;      // 1.0 / (double)(TAU * TAU), the per-event factor of pi^{eta eta} (src/LandauMatch.cpp:365)
   const float *T;
   long t_plane;
   double *out;          // 17 planes of dim doubles (may be null)

@@ -948,7 +948,10 @@ __device__ int landau_cubic(const float t[10], float *e_out, float u_out[4]) {
 }
 
 __device__ float eos_pressure(float e, int eos) {
-  if (eos == 1) return (float)((double)e / 3.0);
+  // (float)((double)e / 3.0) in the reference; e / 3 has a period-2 binary
+  // expansion (or is exact), so rounding it to double first can never land on
+  // a float rounding boundary: the IEEE float quotient is the same number
+  if (eos == 1) return e / 3.0f;
   if (eos == 2) {
     const float e1 = e, e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e, e6 = e5 * e, e7 = e6 * e,
                 e8 = e7 * e, e9 = e8 * e, e10 = e9 * e, e11 = e10 * e, e12 = e11 * e;
@@ -1091,7 +1094,7 @@ __device__ void landau_cell_compute(const EventLandau &ev, bool live, long tidx,
     r.pi[6] = t[6] - u[1] * u[3] * b;
     r.pi[7] = t[7] - u[2] * u[2] * b - c;
     r.pi[8] = t[8] - u[2] * u[3] * b;
-    r.pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * (1.0 / (double)(TAU * TAU)));
+    r.pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * ev.inv_tau2);
   }
   r.e = e;
   r.p = p;
