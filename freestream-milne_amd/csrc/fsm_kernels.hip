@@ -185,17 +185,22 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
 #define SP2_RX (FSMSP2_TX + 2 * FSM_SPLINE_BAND)
 #define SP2_RY (FSMSP_TY + 2 * FSM_SPLINE_BAND)
 #define SP2_P (SP2_RY + 1)  // odd pitch: lanes walking rows do not share banks
+#define SP2_OP (FSMSP_TY + 1)  // staged knots per tile row (float4): rows start in different banks
 // The band weights are read-only for the kernel's lifetime: read through the
 // constant address space, so uniform reads become scalar loads.
 typedef __attribute__((address_space(4))) const float fsm_const_f;
 
-struct SplineLds2 {
-  float zr[SP2_RX * SP2_P];      // z over the tile and the band-wide halo
-  float zxr[FSMSP2_TX * SP2_P];  // z_x over the tile rows and the y halo
+struct __align__(16) SplineLds2 {
+  // z over the tile and the band-wide halo (zr), then z_x over the tile rows
+  // and the y halo (zxr); afterwards the staged knots of the tile
+  float zbuf[SP2_RX * SP2_P + FSMSP2_TX * SP2_P];
+  __device__ float *zr() { return zbuf; }
+  __device__ float *zxr() { return zbuf + SP2_RX * SP2_P; }
   float zmax[4][FSMSP2_TX / FSMSP_TX];
   unsigned bad[FSMSP2_TX / FSMSP_TX];
   float sc;
 };
+static_assert(FSMSP2_TX * SP2_OP * 4 <= SP2_RX * SP2_P + FSMSP2_TX * SP2_P, "the staged knots fit in zbuf");
 
 __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L) {
   constexpr int B = FSM_SPLINE_BAND, W = FSM_SPLINE_W, NF = FSMSP2_TX / FSMSP_TX;
@@ -225,7 +230,7 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
       const int srow = wave + 4 * u;
-      L.zr[srow * SP2_P + lane] = z[u];
+      L.zr()[srow * SP2_P + lane] = z[u];
 #pragma unroll
       for (int i = 0; i < NF; ++i)
         if (srow >= FSMSP_TX * i && srow < FSMSP_TX * i + FSMSP_TX + 2 * B) zm[i] = fmaxf(zm[i], fabsf(z[u]));
@@ -254,7 +259,7 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
     const bool yin = gy >= 0 && gy < ny;
     float win[RA + 2 * B];
 #pragma unroll
-    for (int u = 0; u < RA + 2 * B; ++u) win[u] = L.zr[(RA * wave + u) * SP2_P + lane];
+    for (int u = 0; u < RA + 2 * B; ++u) win[u] = L.zr()[(RA * wave + u) * SP2_P + lane];
     const int r0 = x0 + RA * wave;
     if (r0 >= ev.cxlo && r0 + RA - 1 <= ev.cxhi) {
       fsm_const_f *w = (fsm_const_f *)ev.gx + (long)ev.cxlo * W;  // every row of this wave has these weights
@@ -263,7 +268,7 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
         float acc = 0.0f;
 #pragma unroll
         for (int m = 0; m < W; ++m) acc = fmaf(w[m], win[r + m], acc);
-        L.zxr[(RA * wave + r) * SP2_P + lane] = (yin && r0 + r < nx) ? acc : 0.0f;
+        L.zxr()[(RA * wave + r) * SP2_P + lane] = (yin && r0 + r < nx) ? acc : 0.0f;
       }
     } else {
 #pragma unroll
@@ -274,35 +279,24 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
 #pragma unroll
           for (int m = 0; m < W; ++m) acc = fmaf(w[m], win[r + m], acc);
         }
-        L.zxr[(RA * wave + r) * SP2_P + lane] = (yin && r0 + r < nx) ? acc : 0.0f;
+        L.zxr()[(RA * wave + r) * SP2_P + lane] = (yin && r0 + r < nx) ? acc : 0.0f;
       }
     }
   }
   __syncthreads();
-  // z_y, z_xy for tile row lane (x = x0 + lane) and columns y0 + CB wave + j
+  // z_y, z_xy for tile row lane (x = x0 + lane) and columns y0 + CB wave + j,
+  // staged as knots {z, z_y, z_x, z_xy} in the LDS the windows came from
+  float4 *stg = reinterpret_cast<float4 *>(L.zbuf);  // [FSMSP2_TX rows][SP2_OP]
   {
     float zw[CB + 2 * B], zxw[CB + 2 * B];
 #pragma unroll
     for (int u = 0; u < CB + 2 * B; ++u) {
-      zw[u] = L.zr[(lane + B) * SP2_P + CB * wave + u];
-      zxw[u] = L.zxr[lane * SP2_P + CB * wave + u];
+      zw[u] = L.zr()[(lane + B) * SP2_P + CB * wave + u];
+      zxw[u] = L.zxr()[lane * SP2_P + CB * wave + u];
     }
-    const int gx = x0 + lane;
     const int c0 = y0 + CB * wave;
     const bool canon = c0 >= ev.cylo && c0 + CB - 1 <= ev.cyhi;
-    const int fi = lane / FSMSP_TX;  // this row's flag tile
-    float tol = 0.0f, sc = 0.0f;
-    if (ev.planes_h) {
-      sc = L.sc;
-      float zmx = 0.0f;
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int i = 0; i < NF; ++i)
-          if (i == fi) zmx = fmaxf(zmx, L.zmax[v][i]);
-      tol = -FSM_MFMA_POS_EPS * zmx;
-    }
-    bool bad = false;
+    __syncthreads();  // every window is in registers: the staging may overwrite zr
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
       const int gy = c0 + j;
@@ -314,30 +308,48 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
         zy = fmaf(w[m], zw[j + m], zy);
         zxy = fmaf(w[m], zxw[j + m], zxy);
       }
-      if (gx < nx) {
-        const float z = zw[j + B], zx = zxw[j + B];
-        ev.planes[(long)gx * ny + gy] = make_float4(z, zy, zx, zxy);
+      stg[lane * SP2_OP + CB * wave + j] = make_float4(zw[j + B], zy, zxw[j + B], zxy);
+    }
+  }
+  __syncthreads();
+  // Coalesced writes: element k of the 64 x 32 tile is row k / 32, column
+  // k % 32, so each wave stores two whole 32-knot row runs per instruction.
+  // The fp16 halves and the MFMA-safety test are made here from the staged
+  // knot; a wave's two rows lie in one flag tile.
+  {
+    const float sc = ev.planes_h ? L.sc : 0.0f;
+#pragma unroll
+    for (int q = 0; q < FSMSP2_TX * FSMSP_TY / 256; ++q) {
+      const int k = (int)threadIdx.x + 256 * q;
+      const int row = k / FSMSP_TY, col = k % FSMSP_TY;
+      const int gx = x0 + row, gy = y0 + col;
+      const bool live = gx < nx && gy < ny;
+      bool bad = false;
+      if (live) {
+        const float4 v = stg[row * SP2_OP + col];
+        ev.planes[(long)gx * ny + gy] = v;
         if (ev.planes_h) {
-          const float big = fmaxf(fmaxf(fabsf(z), fabsf(zy)), fmaxf(fabsf(zx), fabsf(zxy))) * sc;
-          bad |= !(knot_cp_min(z, zy, zx, zxy) >= tol) || !(big <= FSM_MFMA_MAXABS);
+          const int fi = row / FSMSP_TX;
+          float zmx = 0.0f;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int i = 0; i < NF; ++i)
+              if (i == fi) zmx = fmaxf(zmx, L.zmax[u][i]);
+          const float tol = -FSM_MFMA_POS_EPS * zmx;
+          const float big = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))) * sc;
+          bad = !(knot_cp_min(v.x, v.y, v.z, v.w) >= tol) || !(big <= FSM_MFMA_MAXABS);
           unsigned short h[4], l[4];
-          split_half(z * sc, h[0], l[0]);
-          split_half(zy * sc, h[1], l[1]);
-          split_half(zx * sc, h[2], l[2]);
-          split_half(zxy * sc, h[3], l[3]);
+          split_half(v.x * sc, h[0], l[0]);
+          split_half(v.y * sc, h[1], l[1]);
+          split_half(v.z * sc, h[2], l[2]);
+          split_half(v.w * sc, h[3], l[3]);
           ev.planes_h[(long)gx * ny + gy] =
               make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
                          l[0] | ((unsigned)l[1] << 16), l[2] | ((unsigned)l[3] << 16));
         }
       }
-    }
-    if (ev.planes_h) {
-      const unsigned long long bm = __ballot(bad);
-      if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < NF; ++i)
-          if ((bm >> (FSMSP_TX * i)) & 0xFFFFull) atomicOr(&L.bad[i], 1u);
-      }
+      if (ev.planes_h && __ballot(bad) != 0ull && lane == 0) atomicOr(&L.bad[row / FSMSP_TX], 1u);
     }
   }
   if (ev.planes_h) {
