@@ -1776,6 +1776,8 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     const fsm_params &p = ev.p;
     if (!ev.plan->is3d) {
       Event2D d = event2d_in_workspace(*ev.plan, p, d_in, path, ev.plan->bvariant[path], cur);
+      // the batch's T is read only by its 3x3 Landau launch (k_landau2d_batch)
+      d.skip_eta_planes = charge ? 0 : 1;
       if (charge) {
         d.charge = 1;
         d.out_j = 1;

@@ -175,6 +175,8 @@ struct Event2D {
   int raw_e;            // 1: the energy input is already the fm^-4 density (IC_ENERGY != 5)
   int out_j;            // 1: write J^tau, J^x, J^y, J^eta (4 planes of T) instead of T^{mu nu}
   float jscale;         // DRAP of calculateBaryonCurrent (src/LandauMatch.cpp:140)
+  int skip_eta_planes;  // 1: T is workspace read only by the 3x3 Landau frame, which does not read
+                        // T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} (zeros): they are not written
 };
 
 // The MFMA form of the 2-D quadrature.  For all samples j sharing a knot

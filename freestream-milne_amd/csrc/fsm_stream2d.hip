@@ -218,13 +218,15 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
       ev.T[0 * P + is] = r[0];
       ev.T[1 * P + is] = r[1];
       ev.T[2 * P + is] = r[2];
-      ev.T[3 * P + is] = 0.0f;
       ev.T[4 * P + is] = r[3];
       ev.T[5 * P + is] = r[4];
-      ev.T[6 * P + is] = 0.0f;
       ev.T[7 * P + is] = r[5];
-      ev.T[8 * P + is] = 0.0f;
-      ev.T[9 * P + is] = 0.0f;
+      if (!ev.skip_eta_planes) {
+        ev.T[3 * P + is] = 0.0f;
+        ev.T[6 * P + is] = 0.0f;
+        ev.T[8 * P + is] = 0.0f;
+        ev.T[9 * P + is] = 0.0f;
+      }
     }
   }
 }
@@ -645,13 +647,15 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     fsm_gstore(ev.T + 0 * P + is, rr[0]);
     fsm_gstore(ev.T + 1 * P + is, rr[1]);
     fsm_gstore(ev.T + 2 * P + is, rr[2]);
-    fsm_gstore(ev.T + 3 * P + is, 0.0f);
     fsm_gstore(ev.T + 4 * P + is, rr[3]);
     fsm_gstore(ev.T + 5 * P + is, rr[4]);
-    fsm_gstore(ev.T + 6 * P + is, 0.0f);
     fsm_gstore(ev.T + 7 * P + is, rr[5]);
-    fsm_gstore(ev.T + 8 * P + is, 0.0f);
-    fsm_gstore(ev.T + 9 * P + is, 0.0f);
+    if (!ev.skip_eta_planes) {
+      fsm_gstore(ev.T + 3 * P + is, 0.0f);
+      fsm_gstore(ev.T + 6 * P + is, 0.0f);
+      fsm_gstore(ev.T + 8 * P + is, 0.0f);
+      fsm_gstore(ev.T + 9 * P + is, 0.0f);
+    }
   }
 }
 
