@@ -69,8 +69,10 @@ def parse():
     ap.add_argument("--latency-samples", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
-    ap.add_argument("--serve-rates", type=str, default="200,1000,3000",
+    ap.add_argument("--serve-rates", type=str, default="50,200,1000,3000,5000",
                     help="offered request rates (per second) of the serving sweep")
+    ap.add_argument("--serve-outstanding", type=int, default=256,
+                    help="requests kept outstanding in the closed-loop (capacity) serving run")
     ap.add_argument("--serve-pool", type=int, default=40, help="distinct requests in the serving pool")
     ap.add_argument("--c3-events", type=int, default=512)
     ap.add_argument("--c3-steps", type=int, default=6)
@@ -127,14 +129,26 @@ def timed(fn, steps, sync):
     return time.perf_counter() - t0
 
 
-def serve_run(fsm_amd, W, ctx, seconds, rate, pool, dev_pool=None, seed=5):
+def _req_class(p, nb):
+    return ("3d" if p.DIM_ETA > 1 else "2d") + ("_baryon" if nb is not None else "")
+
+
+def _pcts(v):
+    v = np.array(v) if len(v) else np.zeros(1)
+    return {"p50": round(float(np.percentile(v, 50)), 4), "p99": round(float(np.percentile(v, 99)), 4),
+            "n": int(len(v))}
+
+
+def serve_run(fsm_amd, W, ctx, seconds, rate, pool, dev_pool=None, seed=5, closed_loop=0):
     """Poisson arrivals at `rate` per second for `seconds` drawn from the C4
     request pool (2-D and 3-D events, some with n_B) through the server;
-    latency = submit to results in HBM.  With dev_pool (the pool's inputs
-    already in HBM) requests are submitted with fsm_server_submit_device,
-    otherwise from host memory."""
+    latency = submit to results in HBM, also per request class.  With
+    dev_pool (the pool's inputs already in HBM) requests are submitted with
+    fsm_server_submit_device, otherwise from host memory.  closed_loop > 0:
+    instead of Poisson arrivals, keep that many requests outstanding (a new one
+    is submitted as each completes): the server's capacity."""
     rng = np.random.default_rng(seed)
-    srv = fsm_amd.Server(ctx, max_batch_events=128, max_inflight=2)
+    srv = fsm_amd.Server(ctx)  # defaults: 3 launch slots (one for 2-D work), 300 us of work per launch
 
     def submit(k):
         p, e, nb = pool[k]
@@ -149,26 +163,41 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, pool, dev_pool=None, seed=5):
         while srv.poll(100000) != rid:
             pass
         srv.release(rid)
+    l0 = srv.stats()["launches"]
     lat, done, busy = [], 0, 0
+    by_class = {}
+    cls = {}
     t0 = time.perf_counter()
     t_next = t0
     submitted = []
+
+    def one():
+        k = int(rng.integers(len(pool)))
+        rid = submit(k)
+        if rid is not None:
+            submitted.append(rid)
+            cls[rid] = _req_class(pool[k][0], pool[k][2])
+        return rid
+
+    if closed_loop:
+        for _ in range(closed_loop):
+            one()
     while True:
         now = time.perf_counter()
-        if now - t0 < seconds and now >= t_next:
-            rid = submit(int(rng.integers(len(pool))))
-            if rid is None:
+        if not closed_loop and now - t0 < seconds and now >= t_next:
+            if one() is None:
                 busy += 1
-            else:
-                submitted.append(rid)
             t_next += rng.exponential(1.0 / rate)
             continue
         rid = srv.poll(50)
         if rid is not None:
             st, l_us, _ = srv.result(rid)
             lat.append(l_us / 1e3)
+            by_class.setdefault(cls.pop(rid), []).append(l_us / 1e3)
             srv.release(rid)
             done += 1
+            if closed_loop and now - t0 < seconds:
+                one()
         elif now - t0 >= seconds and done >= len(submitted):
             break
         if now - t0 > seconds + 30:
@@ -176,13 +205,51 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, pool, dev_pool=None, seed=5):
     el = time.perf_counter() - t0
     stats = srv.stats()
     srv.close()
-    lat = np.array(lat) if lat else np.zeros(1)
-    return {"offered_per_s": rate, "inputs": "host" if dev_pool is None else "device",
-            "completed": done, "refused_busy": busy,
-            "events_per_s": round(done / el, 2),
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                           "p99": round(float(np.percentile(lat, 99)), 4)},
-            "launches": stats["launches"]}
+    out = {"offered_per_s": rate if not closed_loop else None, "inputs": "host" if dev_pool is None else "device",
+           "completed": done, "refused_busy": busy,
+           "events_per_s": round(done / el, 2),
+           "latency_ms": _pcts(lat),
+           "latency_ms_by_class": {k: _pcts(v) for k, v in sorted(by_class.items())},
+           "launches": stats["launches"] - l0}
+    if closed_loop:
+        out["closed_loop_outstanding"] = closed_loop
+    return out
+
+
+def plain_pool_run(fsm_amd, ctx, pool, dev_pool, stream, reps=2):
+    """The plain per-event path (one launch sequence per event, no graph, no
+    batching) on the context's path over every request of the pool, inputs
+    in HBM: events/s, and the device time per request class."""
+    import torch
+
+    outs = [torch.empty((fsm_amd.NUM_OUTPUTS_BARYON if nb is not None else fsm_amd.NUM_OUTPUTS) * p.DIM,
+                        dtype=torch.float64, device="cuda") for p, _, nb in pool]
+    qs = [fsm_amd.params_finalize(p, e) for p, e, _ in pool]
+
+    def run_all():
+        for q, (de, dn), o, (_, _, nb) in zip(qs, dev_pool, outs, pool):
+            if nb is None:
+                ctx.run_device(q, de.data_ptr(), o.data_ptr(), stream)
+            else:
+                ctx.run_baryon_device(q, de.data_ptr(), dn.data_ptr(), o.data_ptr(), stream)
+
+    run_all()
+    torch.cuda.synchronize()
+    t = timed(run_all, reps, torch.cuda.synchronize)
+    per_class = {}
+    for k, (q, (de, dn), o, (p, _, nb)) in enumerate(zip(qs, dev_pool, outs, pool)):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(2):
+            if nb is None:
+                ctx.run_device(q, de.data_ptr(), o.data_ptr(), stream)
+            else:
+                ctx.run_baryon_device(q, de.data_ptr(), dn.data_ptr(), o.data_ptr(), stream)
+        torch.cuda.synchronize()
+        per_class.setdefault(_req_class(p, nb), []).append((time.perf_counter() - a) * 1e3 / 2)
+    del outs
+    return {"events_per_s": round(len(pool) * reps / t, 2),
+            "ms_per_event_by_class": {k: round(float(np.mean(v)), 4) for k, v in sorted(per_class.items())}}
 
 
 def c3_run(fsm_amd, W, ctx, stream, steps, n_events, dev):
@@ -299,12 +366,15 @@ def main():
     info = ctx.plan_info(q1)  # context is on the mixed path here
     samples = D * q1.DIM_PHIP * q1.DIM_RAP * B
     k_ms = mixed["stream_launch_ms"]
-    # MFMA instructions the stream kernel executes per event (the plan mirrors
-    # the kernel's per-wave choice: the knot-displacement form where no mask
-    # applies, the per-offset form at the grid edges), 16*16*32*2 FLOP each
-    # (12 of the 16 columns carry moments)
+    # algorithmic work: the reference's 52 FLOP per (cell, phi) sample (16
+    # spline taps and 10 quadrature rows, 2 FLOP each) over every sample of
+    # the launch, against the peak of the unit the kernel runs on (FP16 MFMA)
+    alg_flops = FLOP_PER_SAMPLE_2D * samples
+    achieved = alg_flops / (k_ms * 1e-3) / 1e12
+    # what the MFMA pipe executes: v_mfma_f32_16x16x32_f16 instructions (the
+    # plan mirrors the kernel's per-wave choice), 16*16*32*2 FLOP each
     mfma_flops = B * info["mfma_batch"] * 16384
-    achieved = mfma_flops / (k_ms * 1e-3) / 1e12
+    executed = mfma_flops / (k_ms * 1e-3) / 1e12
     roofline = {
         "kernel": "k_stream2d_mfma_batch",
         "bound": "mfma",
@@ -314,17 +384,20 @@ def main():
         "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4),
         "traffic": TRAFFIC.get("k_stream2d_mfma_batch"),
         "avg_kernel_ms": round(k_ms, 5),
-        "flop_per_launch": mfma_flops,
-        "fp32_equivalent_tflops": round(FLOP_PER_SAMPLE_2D * samples / (k_ms * 1e-3) / 1e12, 3),
+        "flop_per_launch": alg_flops,
+        "samples_per_launch": samples,
+        "samples_kept_per_launch": int(info["samples_kept"] * B),
+        "achieved_vs_fp32_peak": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "executed_mfma_tflops": round(executed, 3),
+        "executed_mfma_frac": round(executed / FP16_MFMA_PEAK_TFLOPS, 4),
+        "executed_mfma_flop_per_launch": mfma_flops,
         "mfma_per_event": info["mfma_batch"],
-        # the same time against the per-offset form's work (one MFMA per 16 cells and knot offset), the
-        # measure this kernel was quoted in before the knot-displacement form cut the instruction count
-        "offset_form_equivalent_tflops": round(B * D * info["n_offsets"] * 1024 / (k_ms * 1e-3) / 1e12, 3),
-        "note": "v_mfma_f32_16x16x32_f16 instructions the kernel executes: %d events x %d per event x 16384 FLOP "
-                "per launch (per 16 cells: %.1f, against %d knot offsets of the per-offset form), against "
-                "MI355X's dense FP16 peak; fp32_equivalent counts the reference's 52 FLOP per (cell, phi) sample "
-                "over the same time; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE "
-                "(profiles/)" % (B, info["mfma_batch"], info["mfma_batch"] * 16.0 / D, info["n_offsets"]),
+        "note": "achieved = the reference's 52 FLOP per (cell, phi) sample x %d samples per launch (%d events) / "
+                "the launch's average time (HIP events on the launch stream), against MI355X's dense FP16 MFMA "
+                "peak; executed_mfma counts the v_mfma_f32_16x16x32_f16 instructions the kernel issues (%d per "
+                "event, 16384 FLOP each: hi+lo triple product, 12 of 16 columns used), %.1fx the algorithmic "
+                "work; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"
+                % (samples, B, info["mfma_batch"], mfma_flops / alg_flops),
     }
     kf = fp32["stream_launch_ms"]
     roofline_fp32 = {
@@ -354,6 +427,7 @@ def main():
         d2 = torch.from_numpy(e2).to(dev)
         o2 = torch.empty(17 * q2.DIM, dtype=torch.float64, device=dev)
         samples3 = q2.DIM * q2.DIM_RAP * q2.DIM_PHIP
+        info3 = ctx.plan_info(q2)
         c2 = {"workload": "3-D 131x131x101, DIM_RAP=21, DIM_PHIP=48->51, DTAU=0.8, SIGMA=1; one event per call, "
                           "device-resident"}
         e_by_path = {}
@@ -380,14 +454,23 @@ def main():
                                        "p99": round(float(np.percentile(lat2, 99)), 4)},
                         "stage_ms": {"src": round(kt2[0] / n2, 4), "stream": round(kt2[1] / n2, 4),
                                      "landau": round(kt2[2] / n2, 4)},
-                        "stream_tflops_reference_samples":
-                            round(FLOP_PER_SAMPLE_3D * samples3 / (kt2[1] / n2 * 1e-3) / 1e12, 3)}
+                        # useful work: 36 FLOP per (cell, rapidity, phi) sample the masks keep
+                        "stream_tflops_kept_samples":
+                            round(FLOP_PER_SAMPLE_3D * info3["samples_kept"] / (kt2[1] / n2 * 1e-3) / 1e12, 3),
+                        "stream_frac_of_fp32_peak_kept_samples":
+                            round(FLOP_PER_SAMPLE_3D * info3["samples_kept"] / (kt2[1] / n2 * 1e-3) / 1e12 /
+                                  FP32_PEAK_TFLOPS, 4),
+                        "landau_hbm_tb_per_s":
+                            round(q2.DIM * (40 + 136) / (kt2[2] / n2 * 1e-3) / 1e12, 3)}
         c2["mixed_over_fp32"] = round(c2["mixed"]["events_per_s"] / c2["fp32"]["events_per_s"], 3)
         c2["mixed_vs_fp32_max_e_diff_rel"] = float(np.abs(e_by_path["mixed"] - e_by_path["fp32"]).max() /
                                                    np.abs(e_by_path["fp32"]).max())
+        c2["samples"] = {"nominal": samples3, "kept": int(info3["samples_kept"]),
+                         "evaluated": int(info3["samples_evaluated"])}
         c2["note"] = ("3-D mixed path = the FP32 stream kernel + FP32/FP64 Landau (an FP16-MFMA form of the 3-D "
                       "quadrature measured slower and is not built); stream_tflops counts the reference's 36 FLOP "
-                      "over all DIM*DIM_RAP*DIM_PHIP samples, most of which the masks drop")
+                      "per sample the masks keep (kept of nominal DIM*DIM_RAP*DIM_PHIP; evaluated = what the "
+                      "kernel computes); landau_hbm = 40 B in + 136 B out per cell over the Landau stage time")
         del d2, o2
 
     # ---- C3: a batch of 512 mixed 2-D events --------------------------------------
@@ -406,15 +489,27 @@ def main():
         dev_pool = [(torch.from_numpy(e).to(dev), None if b is None else torch.from_numpy(b).to(dev))
                     for _, e, b in pool]
         sync()
-        sweep = [serve_run(fsm_amd, W, ctx, args.serve_seconds, float(r), pool, dev_pool)
-                 for r in args.serve_rates.split(",")]
-        host_point = serve_run(fsm_amd, W, ctx, args.serve_seconds, float(args.serve_rates.split(",")[1]), pool)
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        sweep = [serve_run(fsm_amd, W, ctx, args.serve_seconds, r, pool, dev_pool) for r in rates]
+        capacity = serve_run(fsm_amd, W, ctx, args.serve_seconds, 0.0, pool, dev_pool,
+                             closed_loop=args.serve_outstanding)
+        host_point = serve_run(fsm_amd, W, ctx, args.serve_seconds, 1000.0, pool)
+        # the plain FP32 path (one event per launch sequence, no batching) over the same pool
+        ctx.set_path(fsm_amd.PATH_FP32)
+        plain = plain_pool_run(fsm_amd, ctx, pool, dev_pool, stream)
+        ctx.set_path(fsm_amd.PATH_MIXED)
+        plain_mixed = plain_pool_run(fsm_amd, ctx, pool, dev_pool, stream)
         del dev_pool
         serve = {"workload": "Poisson arrivals for %.1f s per rate from a pool of %d requests: %d 3-D (131^2 x "
                              "DIM_ETA 41-101, DIM_RAP 21), the rest C3-style 2-D; %d carry n_B (BARYON=1, "
                              "SIGMA_B=0.8); profiles resident in HBM (submit_device), one rate also from host "
                              "memory; results kept in HBM; mixed path" % (args.serve_seconds, len(pool), n3, nb),
-                 "sweep": sweep, "host_inputs": host_point}
+                 "sweep": sweep, "capacity_closed_loop": capacity, "host_inputs": host_point,
+                 "plain_fp32_same_pool": plain, "plain_mixed_same_pool": plain_mixed,
+                 "serving_over_plain_fp32": round(capacity["events_per_s"] / plain["events_per_s"], 3),
+                 "note": "serving_over_plain_fp32 = the server's closed-loop capacity on the mixed path over the "
+                         "plain per-event FP32 path on the same pool; the pool's device time is dominated by its "
+                         "3-D events, whose stream kernel is FP32 on both paths"}
 
     # ---- CPU baseline (oracle float path = the reference algorithm) --------------
     cpu = None
@@ -423,14 +518,31 @@ def main():
 
         qo = O.params_finalize(p1, e1)
         n = 0
+        stage = np.zeros(4)
         a = time.perf_counter()
         while True:
-            O.run(qo, e1, "float", want_T=False, want_extra=False)
+            r = O.run(qo, e1, "float", want_T=False, want_extra=False)
+            stage += r["stage_seconds"]
             n += 1
             if time.perf_counter() - a >= args.cpu_seconds:
                 break
         cs = time.perf_counter() - a
+        model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": round(n / cs, 4), "unit": "events/s", "cores": O.num_threads(), "kind": "port",
+               "cpu_model": model, "host_cpus": os.cpu_count(),
+               # per event, at the reference's own timer points (src/FreestreamMilne.cpp:375-393 freeStream,
+               # :416-424 stress tensor, :475-482 eigen-solve; the output pack :677-718)
+               "stage_seconds_per_event": {"convert_and_freestream": round(stage[0] / n, 5),
+                                           "hypertrig_and_stress_tensor": round(stage[1] / n, 5),
+                                           "eigen_eos_bulk_shear": round(stage[2] / n, 5),
+                                           "output_pack": round(stage[3] / n, 5)},
                "sample": "%d C1 event(s) through the oracle's reference float path (unfused shiftedDensity, "
                          "10-pass quadrature, per-cell eigen-solve), OpenMP, %.1f s" % (n, cs)}
 

@@ -149,6 +149,9 @@ struct Plan {
   DevBuf<int> eta_order;
   DevBuf<int2> srange;
   std::vector<float> eta_work;  // kept cell-samples of each eta plane (work estimate)
+  // (cell, sample) pairs the reference's masks keep, and (3-D) the pairs the
+  // stream kernel evaluates (whole tiles per sample whose rows it reaches)
+  double samples_kept = 0.0, samples_evaluated = 0.0;
   DevBuf<float> src;
   DevBuf<float> Tem;  // eta-major T
   // charge pass (BARYON = 1): normalisation of convertInitialChargeDensity;
@@ -208,6 +211,13 @@ struct fsm_ctx {
   // host-API workspace
   DevBuf<float> d_in, d_nb, d_T, d_J;
   DevBuf<double> d_out;
+  // One-event calls share the workspace above and the plan's event-sized
+  // buffers (spline planes, MFMA planes, 3-D source planes, eta-major T), so
+  // they are ordered: a call's stream first waits for the previous call's
+  // work (on whatever stream it was queued), and host threads take turns.
+  std::recursive_mutex call_mu;  // fsm_run holds it around its nested fsm_run_device
+  hipEvent_t last_call = nullptr;
+  bool last_call_recorded = false;
 };
 
 namespace {
@@ -480,6 +490,16 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     S.ss = s * s;
     S.dxj = dxj;
     S.dyj = dyj;
+  }
+  {
+    // cells whose sample j the reference keeps: floor index ix + ox in [1, nx-2]
+    // (a sample whose shift is within rounding of a whole cell may differ by
+    // one row of cells; that does not matter for a work count)
+    auto kept_run = [](int n, int o) { return (double)std::max(0, std::min(n - 2 - o, n - 1) - std::max(1 - o, 0) + 1); };
+    pl.samples_kept = 0.0;
+    for (int j = 0; j < nphi; ++j) pl.samples_kept += kept_run(nx, st[j].ox) * kept_run(ny, st[j].oy);
+    pl.samples_kept *= p.DIM_RAP;  // identical rapidity copies in 2-D
+    pl.samples_evaluated = (double)nx * ny * nphi;
   }
   // Group contiguous azimuth ranges so that each group's knot box fits the
   // LDS budget (3 workgroups per CU).
@@ -756,11 +776,15 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   // work per eta plane: kept cell-samples of its slots; the one-event launch
   // starts the heaviest planes first
   pl.eta_work.assign(neta, 0.0f);
+  pl.samples_kept = 0.0;
   for (int ie = 0; ie < neta; ++ie)
     for (int k = 0; k < nrap; ++k) {
       const RapSlot &S = slots[(size_t)ie * nrap + k];
-      for (int j = S.s0; j < S.s0 + S.ns; ++j)
-        pl.eta_work[ie] += (float)(samp[j].xhi - samp[j].xlo + 1) * (float)(samp[j].yhi - samp[j].ylo + 1);
+      for (int j = S.s0; j < S.s0 + S.ns; ++j) {
+        const double kept = (double)(samp[j].xhi - samp[j].xlo + 1) * (double)(samp[j].yhi - samp[j].ylo + 1);
+        pl.eta_work[ie] += (float)kept;
+        pl.samples_kept += kept;
+      }
     }
   {
     std::vector<int> order(neta);
@@ -773,6 +797,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     // cell in the tile's rows (tiles are FSM3D_TILE consecutive cells)
     const int tiles = (nx * ny + FSM3D_TILE - 1) / FSM3D_TILE;
     std::vector<int2> rg((size_t)neta * nrap * tiles, make_int2(0, 0));
+    pl.samples_evaluated = 0.0;
     for (size_t si = 0; si < slots.size(); ++si) {
       const RapSlot &S = slots[si];
       for (int t = 0; t < tiles; ++t) {
@@ -784,6 +809,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
           if (std::max(rlo, P.xlo) <= std::min(rhi, P.xhi)) {
             if (first < 0) first = j;
             last = j;
+            pl.samples_evaluated += FSM3D_TILE;  // every cell of the tile, dropped ones read zeros
           }
         }
         if (first >= 0) rg[si * tiles + t] = make_int2(first, last + 1);
@@ -1028,11 +1054,36 @@ void finish_timing(fsm_ctx *ctx) {
 
 int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p);  // batch section
 
+// Orders a one-event call after the previous one on the same context (see
+// fsm_ctx::call_mu).  A stream under graph capture is left alone: capture
+// cannot wait on work recorded outside it, and the caller's graph orders
+// its own nodes.
+struct CallOrder {
+  fsm_ctx *ctx;
+  hipStream_t s;
+  bool capturing = false;
+  std::lock_guard<std::recursive_mutex> lock;
+  CallOrder(fsm_ctx *c, hipStream_t st) : ctx(c), s(st), lock(c->call_mu) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cs) == hipSuccess) capturing = cs != hipStreamCaptureStatusNone;
+  }
+  hipError_t begin() {
+    if (capturing || !ctx->last_call_recorded) return hipSuccess;
+    return hipStreamWaitEvent(s, ctx->last_call, 0);
+  }
+  ~CallOrder() {
+    if (capturing || !ctx->last_call) return;
+    if (hipEventRecord(ctx->last_call, s) == hipSuccess) ctx->last_call_recorded = true;
+  }
+};
+
 int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double *d_out, float *T_scratch,
                     hipStream_t s, const float *d_nb = nullptr, float *J_scratch = nullptr) {
-  Plan *pl = nullptr;
-  int st = get_plan(ctx, *p, &pl);
+  // held for the call: the server thread may evict it from the cache meanwhile
+  std::shared_ptr<Plan> plan;
+  int st = get_plan_shared(ctx, *p, &plan);
   if (st) return st;
+  Plan *pl = plan.get();
   const float *T = nullptr;
   int em = 0;
   st = stage_a(ctx, *pl, *p, d_e, T_scratch, s, &T, &em);
@@ -1091,6 +1142,7 @@ int fsm_create(int device, fsm_ctx **out) {
   ctx->max_lds = lds;
   ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   FSM_TRY(fsm_set_lds_limits(lds, lds));
+  FSM_TRY(hipEventCreateWithFlags(&ctx->last_call, hipEventDisableTiming));
   *out = ctx.release();
   return FSM_OK;
 }
@@ -1100,6 +1152,7 @@ void fsm_destroy(fsm_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   for (auto &set : ctx->pool)
     for (auto &e : set) (void)hipEventDestroy(e);
+  if (ctx->last_call) (void)hipEventDestroy(ctx->last_call);
   delete ctx;
 }
 
@@ -1156,6 +1209,8 @@ int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info) {
   if (st) return st;
   memset(info, 0, sizeof(*info));
   info->dim3 = pl->is3d ? 1 : 0;
+  info->samples_kept = pl->samples_kept;
+  info->samples_evaluated = pl->samples_evaluated;
   if (pl->is3d) {
     info->tiles = pl->ev3.tiles * pl->ev3.neta;
     info->n_phi = pl->ev3.nphi;
@@ -1249,6 +1304,8 @@ int fsm_sync(fsm_ctx *ctx, void *stream) {
 int fsm_run_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, double *d_out, void *stream) {
   if (!ctx || !p || !d_e_GeV || !d_out) return FSM_ERR_ARG;
   FSM_TRY(hipSetDevice(ctx->device));
+  CallOrder order(ctx, (hipStream_t)stream);
+  FSM_TRY(order.begin());
   float *Tscr = nullptr;
   if (p->DIM_ETA == 1) {
     FSM_TRY(ctx->d_T.ensure((size_t)10 * p->DIM));
@@ -1260,10 +1317,13 @@ int fsm_run_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, doub
 int fsm_stream_tmunu_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, float *d_T, void *stream) {
   if (!ctx || !p || !d_e_GeV || !d_T) return FSM_ERR_ARG;
   FSM_TRY(hipSetDevice(ctx->device));
-  Plan *pl = nullptr;
-  int st = get_plan(ctx, *p, &pl);
-  if (st) return st;
   hipStream_t s = (hipStream_t)stream;
+  CallOrder order(ctx, s);
+  FSM_TRY(order.begin());
+  std::shared_ptr<Plan> plan;
+  int st = get_plan_shared(ctx, *p, &plan);
+  if (st) return st;
+  Plan *pl = plan.get();
   const float *T = nullptr;
   int em = 0;
   st = stage_a(ctx, *pl, *p, d_e_GeV, d_T, s, &T, &em);
@@ -1283,6 +1343,8 @@ int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T, float
   int st = check_supported(*p);
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
+  CallOrder order(ctx, s);  // the timing events are the context's too
+  FSM_TRY(order.begin());
   EventLandau L = landau_event(*p, d_T, 0, ctx->path == FSM_PATH_MIXED);
   L.fe = d_e;
   L.fu = d_u;
@@ -1306,6 +1368,7 @@ int fsm_run(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, size_t n, double *c
   if (st) return st;
   if (tau_landau_match) *tau_landau_match = p->TAU;
   FSM_TRY(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);  // ctx->d_in, d_out
   const size_t D = (size_t)p->DIM;
   FSM_TRY(ctx->d_in.ensure(D));
   FSM_TRY(ctx->d_out.ensure(D * FSM_NUM_OUTPUTS));
@@ -1326,6 +1389,8 @@ int fsm_run_baryon_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_Ge
   int st = check_supported(*p);
   if (st) return st;
   FSM_TRY(hipSetDevice(ctx->device));
+  CallOrder order(ctx, (hipStream_t)stream);
+  FSM_TRY(order.begin());
   float *Tscr = nullptr, *Jscr = nullptr;
   if (p->DIM_ETA == 1) {
     FSM_TRY(ctx->d_T.ensure((size_t)10 * p->DIM));
@@ -1348,6 +1413,7 @@ int fsm_run_baryon(fsm_ctx *ctx, fsm_params *p, const float *e_GeV, const float 
   if (st) return st;
   if (tau_landau_match) *tau_landau_match = p->TAU;
   FSM_TRY(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);  // ctx->d_in, d_nb, d_out
   const size_t D = (size_t)p->DIM;
   FSM_TRY(ctx->d_in.ensure(D));
   FSM_TRY(ctx->d_nb.ensure(D));
@@ -1741,14 +1807,26 @@ struct fsm_batch {
   // pinned staging with hipMemcpyAsync on the launch stream
   bool async_upload = false;
   PinnedBuf staging;
+  // completion of the last fsm_batch_run: rebuilding the descriptors or the
+  // graph waits for it, since that run may still read them on its stream
+  hipEvent_t last_run = nullptr;
+  bool last_run_recorded = false;
 
+  void wait_last_run() {
+    if (last_run_recorded) (void)hipEventSynchronize(last_run);
+    last_run_recorded = false;
+  }
   void drop_graph() {
+    wait_last_run();
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
   }
-  ~fsm_batch() { drop_graph(); }
+  ~fsm_batch() {
+    drop_graph();
+    if (last_run) (void)hipEventDestroy(last_run);
+  }
 };
 
 namespace {
@@ -1897,6 +1975,8 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   return FSM_OK;
 }
 
+int batch_run_graph_or_direct(fsm_batch *b, hipStream_t s);
+
 int batch_launch(fsm_batch *b, hipStream_t s, bool timed) {
   fsm_ctx *ctx = b->ctx;
   const int4 *it = b->d_items.p;
@@ -1982,10 +2062,22 @@ int fsm_batch_run(fsm_batch *b, void *stream) {
   if (b->evs.empty()) return FSM_OK;
   FSM_TRY(hipSetDevice(b->ctx->device));
   hipStream_t s = (hipStream_t)stream;
+  if (!b->last_run) FSM_TRY(hipEventCreateWithFlags(&b->last_run, hipEventDisableTiming));
   if (b->dirty || b->built_path != b->ctx->path) {
     int st = batch_build(b, s);
     if (st) return st;
   }
+  int st = batch_run_graph_or_direct(b, s);
+  if (st) return st;
+  FSM_TRY(hipEventRecord(b->last_run, s));
+  b->last_run_recorded = true;
+  return FSM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+int batch_run_graph_or_direct(fsm_batch *b, hipStream_t s) {
   if (!b->use_graph || s == nullptr) return batch_launch(b, s, b->ctx->timing != 0);
   if (!b->exec) {
     FSM_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -2003,8 +2095,7 @@ int fsm_batch_run(fsm_batch *b, void *stream) {
   FSM_TRY(hipGraphLaunch(b->exec, s));
   return FSM_OK;
 }
-
-}  // extern "C"
+}  // namespace
 
 // ---- internal entry points for the server (fsm_host.h) ----------------------
 namespace fsm_host {
@@ -2030,11 +2121,33 @@ int batch_launch_now(fsm_batch *b, hipStream_t s) {
 
 int ctx_device(const fsm_ctx *ctx) { return ctx->device; }
 
-int plan_check(fsm_ctx *ctx, const fsm_params *p) {
+// Batched device time per unit of work on MI355X, from the measured stage
+// times of the 261^2 lumpy event (64-event batches) and of the 131^2 x 101
+// event: 2-D stream per (cell, phi) sample on each path, 3-D stream per
+// evaluated sample, spline / source planes and Landau per cell.
+const float kCostStream2dNs[2] = {1.06f, 0.62f};  // FP32, mixed
+const float kCostStream3dNs = 1.10f;
+const float kCostPlanes2dNs = 0.022f, kCostPlanes3dNs = 0.010f;
+const float kCostLandau2dNs = 0.044f, kCostLandau3dNs = 0.081f;
+
+int plan_check(fsm_ctx *ctx, const fsm_params *p, bool baryon, float *cost_us) {
   std::shared_ptr<Plan> sp;
   int st = get_plan_shared(ctx, *p, &sp);
   if (st) return st;
-  return maybe_tune(ctx, *sp, *p);
+  st = maybe_tune(ctx, *sp, *p);
+  if (st) return st;
+  if (cost_us) {
+    const int passes = baryon ? 2 : 1;  // energy and charge streaming
+    const double D = (double)p->DIM;
+    double ns;
+    if (sp->is3d)
+      ns = passes * (sp->samples_evaluated * kCostStream3dNs + D * kCostPlanes3dNs) + D * kCostLandau3dNs;
+    else
+      ns = passes * (sp->samples_evaluated * kCostStream2dNs[ctx->path == FSM_PATH_MIXED ? 1 : 0] +
+                     D * kCostPlanes2dNs) + D * kCostLandau2dNs;
+    *cost_us = (float)(ns * 1e-3);
+  }
+  return FSM_OK;
 }
 
 }  // namespace fsm_host

@@ -21,8 +21,10 @@ void batch_set_async_upload(fsm_batch *b, bool on);
 // Builds (if changed) and launches the batch on `s` without a graph.
 int batch_launch_now(fsm_batch *b, hipStream_t s);
 int ctx_device(const fsm_ctx *ctx);
-// Builds/looks up the plan for p (returns its status: unsupported shapes fail here).
-int plan_check(fsm_ctx *ctx, const fsm_params *p);
+// Builds/looks up the plan for p (returns its status: unsupported shapes fail
+// here) and estimates the event's device time inside a batch on the
+// context's path, in microseconds (cost_us may be NULL).
+int plan_check(fsm_ctx *ctx, const fsm_params *p, bool baryon, float *cost_us);
 
 }  // namespace fsm_host
 

@@ -1197,7 +1197,10 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds,
   const bool live = xy < ev.nxy && ie < ev.neta;
   const int np = ev.J ? FSM_NUM_OUTPUTS + 5 : FSM_NUM_OUTPUTS;
   CellOut r;
-  landau_cell_compute<4>(ev, live, (long)ie * ev.nxy + xy, r, S);  // starts with a barrier
+  // On the mixed path this starts with a barrier; on the FP32 path it has
+  // none, so the trailing barrier below is what keeps the staged outputs of
+  // one tile from being overwritten by the next tile of the same workgroup.
+  landau_cell_compute<4>(ev, live, (long)ie * ev.nxy + xy, r, S);
   if (live) {
     const int li = xl * FSML_PITCH + el;
 #pragma unroll
@@ -1217,6 +1220,7 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds,
       o[is] = plane_double(ev, lds[c * FSML_PLANE + li], c);
     }
   }
+  __syncthreads();  // every lane has read its staged outputs
 }
 
 template <int N>

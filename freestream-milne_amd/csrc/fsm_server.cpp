@@ -7,12 +7,16 @@
 // request's HBM footprint against the budget, copies the input to pinned
 // staging and queues an asynchronous upload on the upload stream.  A worker
 // thread owns the launch slots (one non-blocking stream, batch and completion
-// event each).  Whenever a slot is free and requests are queued, it takes up
-// to max_batch_events of them, makes the slot's stream wait for their
-// uploads, and launches them as one batch (one launch per stage); it retires
-// slots whose completion event has fired.  Requests that arrive while
-// batches run are therefore admitted and launched next to the running work.
-// Results stay in HBM until fetched or released.
+// event each).  Whenever a slot is free and requests are queued, it takes
+// queued requests in arrival order while their estimated device time (from
+// the event's plan) stays within max_batch_us and their number within
+// max_batch_events, makes the slot's stream wait for their uploads, and
+// launches them as one batch (one launch per stage); it retires slots whose
+// completion event has fired.  With two or more slots, slot 0 takes only 2-D
+// requests, so a 2-D request never waits behind a launch of 3-D events (each
+// of which is about a millisecond of device time).  Requests that arrive
+// while batches run are therefore admitted and launched next to the running
+// work.  Results stay in HBM until fetched or released.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -92,6 +96,8 @@ struct Request {
   fsm_params p{};
   size_t in_bytes = 0, out_bytes = 0, ws_bytes = 0, footprint = 0;
   bool baryon = false;   // input e then n_B; output FSM_NUM_OUTPUTS_BARYON planes
+  bool small = true;     // a 2-D event (may use the slot kept for 2-D work)
+  float cost_us = 0.0f;  // estimated device time inside a batch
   float *d_in = nullptr;
   double *d_out = nullptr;
   void *ws = nullptr;
@@ -171,23 +177,40 @@ void worker_loop(fsm_server *s) {
       sl.busy = false;
       s->cv_done.notify_all();
     }
-    // launch into a free slot
+    // compose a launch for the first free slot that has eligible work
     Slot *free_slot = nullptr;
-    for (auto &sl : s->slots)
-      if (!sl.busy) {
-        free_slot = &sl;
-        break;
-      }
-    if (free_slot && !s->queued.empty()) {
-      std::vector<Request *> take;
-      while (!s->queued.empty() && (int)take.size() < s->cfg.max_batch_events) {
-        auto it = s->reqs.find(s->queued.front());
+    std::vector<Request *> take;
+    for (size_t si = 0; si < s->slots.size() && take.empty() && !s->queued.empty(); ++si) {
+      if (s->slots[si].busy) continue;
+      const bool small_only = s->slots.size() >= 2 && si == 0;
+      std::deque<int64_t> skipped;  // stay queued, in order
+      float work = 0.0f;
+      while (!s->queued.empty()) {
+        const int64_t id = s->queued.front();
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end() || it->second->state != REQ_QUEUED) {
+          s->queued.pop_front();
+          continue;
+        }
+        Request *r = it->second;
+        if (small_only && !r->small) {
+          skipped.push_back(id);
+          s->queued.pop_front();
+          continue;
+        }
+        if ((int)take.size() >= s->cfg.max_batch_events ||
+            (!take.empty() && work + r->cost_us > s->cfg.max_batch_us))
+          break;  // the launch is full; later requests keep their order
         s->queued.pop_front();
-        if (it == s->reqs.end() || it->second->state != REQ_QUEUED) continue;
-        it->second->state = REQ_RUNNING;
-        take.push_back(it->second);
+        r->state = REQ_RUNNING;
+        work += r->cost_us;
+        take.push_back(r);
       }
-      if (!take.empty()) {
+      s->queued.insert(s->queued.begin(), skipped.begin(), skipped.end());
+      if (!take.empty()) free_slot = &s->slots[si];
+    }
+    if (free_slot) {
+      {
         Slot &sl = *free_slot;
         sl.busy = true;
         sl.ids.clear();
@@ -234,8 +257,9 @@ int fsm_server_create(fsm_ctx *ctx, const fsm_server_config *cfg, fsm_server **o
   if (cfg) s->cfg = *cfg;
   if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = (size_t)16 << 30;
   if (s->cfg.max_batch_events <= 0) s->cfg.max_batch_events = 256;
-  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
+  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 3;
   if (s->cfg.max_requests <= 0) s->cfg.max_requests = 4096;
+  if (!(s->cfg.max_batch_us > 0.0f)) s->cfg.max_batch_us = 300.0f;
   int st = FSM_OK;
   if (hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking) != hipSuccess) st = FSM_ERR_DEVICE;
   s->slots.resize(s->cfg.max_inflight);
@@ -308,12 +332,15 @@ int64_t submit_impl(fsm_server *s, const fsm_params *p, const float *e_host, boo
   // BARYON = 1 needs n_B; with BARYON = 0 an n_B is ignored, as the reference ignores charge then
   if (q.BARYON && !have_nb) return FSM_ERR_ARG;
   const bool baryon = q.BARYON && have_nb;
-  st = fsm_host::plan_check(s->ctx, &q);
+  float cost_us = 0.0f;
+  st = fsm_host::plan_check(s->ctx, &q, baryon, &cost_us);
   if (st) return st;
   Request *r = new (std::nothrow) Request();
   if (!r) return FSM_ERR_OOM;
   r->p = q;
   r->baryon = baryon;
+  r->small = q.DIM_ETA == 1;
+  r->cost_us = cost_us;
   r->in_bytes = (size_t)q.DIM * sizeof(float) * (baryon ? 2 : 1);
   r->out_bytes = (size_t)q.DIM * (baryon ? FSM_NUM_OUTPUTS_BARYON : FSM_NUM_OUTPUTS) * sizeof(double);
   r->ws_bytes = fsm_host::workspace_bytes(q, baryon);

@@ -102,7 +102,8 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("dim3", "variant", "tile_w", "tile_h", "tiles", "n_phi", "n_offsets",
                                             "n_offset_groups", "n_vector_samples", "lds_bytes", "batch_variant",
                                             "batch_tiles", "tuned")] + [("tune_us", ctypes.c_float * 3)] + \
-        [(n, ctypes.c_int) for n in ("mfma_batch", "mfma_one")]
+        [(n, ctypes.c_int) for n in ("mfma_batch", "mfma_one")] + \
+        [(n, ctypes.c_double) for n in ("samples_kept", "samples_evaluated")]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -114,7 +115,7 @@ class ServerConfig(ctypes.Structure):
     """fsm_server_config (include/fsm.h)."""
 
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_events", ctypes.c_int),
-                ("max_inflight", ctypes.c_int), ("max_requests", ctypes.c_int)]
+                ("max_inflight", ctypes.c_int), ("max_requests", ctypes.c_int), ("max_batch_us", ctypes.c_float)]
 
 
 class RunReport(ctypes.Structure):
@@ -521,9 +522,9 @@ class Server:
     them to host and releases the request."""
 
     def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, max_batch_events: int = 0,
-                 max_inflight: int = 0, max_requests: int = 0):
+                 max_inflight: int = 0, max_requests: int = 0, max_batch_us: float = 0.0):
         self._ctx = ctx
-        cfg = ServerConfig(hbm_budget_bytes, max_batch_events, max_inflight, max_requests)
+        cfg = ServerConfig(hbm_budget_bytes, max_batch_events, max_inflight, max_requests, max_batch_us)
         self._s = ctypes.c_void_p()
         _check(_lib().fsm_server_create(ctx._ctx, ctypes.byref(cfg), ctypes.byref(self._s)), "fsm_server_create")
         self._dims = {}

@@ -111,11 +111,12 @@ def config(name: str, seed: int | None = None):
     raise KeyError(name)
 
 
-def c2_like(neta: int = 101, seed: int = 2):
-    """A C2-style 3-D event with DIM_ETA = neta (the 3-D requests of C4)."""
-    p = base_params(DIM_X=131, DIM_Y=131, DIM_ETA=neta, DIM_RAP=21, DIM_PHIP=48, DX=0.1, DY=0.1,
+def c2_like(neta: int = 101, seed: int = 2, nx: int = 131):
+    """A C2-style 3-D event with DIM_ETA = neta (the 3-D requests of C4);
+    nx < 131 keeps the central nx^2 window of the same kind of profile."""
+    p = base_params(DIM_X=nx, DIM_Y=nx, DIM_ETA=neta, DIM_RAP=21, DIM_PHIP=48, DX=0.1, DY=0.1,
                     DETA=0.1, TAU0=0.1, DTAU=0.8, SIGMA=1.0)
-    e2 = hotspots_2d(131, 131, 0.1, 0.1, 100, seed)
+    e2 = hotspots_2d(nx, nx, 0.1, 0.1, 100, seed)
     env = eta_envelope(neta, 0.1)
     return p, (e2[:, :, None] * env[None, None, :]).astype(np.float32).ravel()
 

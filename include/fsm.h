@@ -151,6 +151,8 @@ typedef struct fsm_plan_info {
   float tune_us[3];     /* autotuner: stream time per event of each shape */
   int mfma_batch;       /* MFMA instructions (16x16x32) of one event's stream stage on the batch shape */
   int mfma_one;         /* the same on the one-event shape (both count every tile on the MFMA form) */
+  double samples_kept;      /* (cell, rapidity, azimuth) samples the reference's masks keep */
+  double samples_evaluated; /* samples the stream kernel evaluates (3-D: whole tiles per reached sample) */
 } fsm_plan_info;
 int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info);
 
@@ -214,7 +216,15 @@ int fsm_run_baryon_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_Ge
 /* Device-resident entry points.  All pointers are device pointers; `stream`
  * is a hipStream_t (NULL = default stream).  `p` must be finalized.  Calls
  * are asynchronous on `stream`; tables for the event shape are built on the
- * first call with that shape and cached in the context. */
+ * first call with that shape and cached in the context.
+ *
+ * Ordering: the one-event calls of a context share its workspace (T^{mu nu}
+ * scratch, spline planes, 3-D source planes), so each call's stream first
+ * waits for the previous call's work on that context, whatever stream it was
+ * queued on, and concurrent host threads take turns.  Two calls on different
+ * streams therefore do not overlap on the device; for overlapping events use
+ * a batch, the server, or one context per stream.  A stream under graph
+ * capture is not made to wait (the capturing caller orders its graph). */
 
 /* Whole step: e_GeV[DIM] -> out[17*DIM] doubles (SoA, FSM_NUM_OUTPUTS planes
  * of DIM in the output order).  Equivalent to fsm_run without host copies. */
@@ -248,7 +258,8 @@ int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T,
  * in GeV/fm^3, output 17*DIM doubles, as fsm_run_device) and returns the
  * event's index (>= 0) or a negative status; the pointers must stay valid
  * until fsm_batch_clear.  Workspace is allocated by the batch on its first
- * run after a change. */
+ * run after a change; that rebuild (and fsm_batch_clear, and destroy) waits
+ * for the batch's previous fsm_batch_run to finish on its stream. */
 typedef struct fsm_batch fsm_batch;
 int fsm_batch_create(fsm_ctx *ctx, fsm_batch **out);
 void fsm_batch_destroy(fsm_batch *b);
@@ -278,8 +289,10 @@ typedef struct fsm_server fsm_server;
 typedef struct fsm_server_config {
   size_t hbm_budget_bytes; /* device memory for admitted requests (0: 16 GiB) */
   int max_batch_events;    /* events per launch (0: 256) */
-  int max_inflight;        /* launches in flight (0: 2) */
+  int max_inflight;        /* launches in flight (0: 3) */
   int max_requests;        /* admitted + completed-unreleased requests (0: 4096) */
+  float max_batch_us;      /* estimated device time per launch, microseconds (0: 300); a
+                              launch always takes at least one request */
 } fsm_server_config;
 
 int fsm_server_create(fsm_ctx *ctx, const fsm_server_config *cfg, fsm_server **out);

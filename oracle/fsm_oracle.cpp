@@ -433,12 +433,34 @@ void landau_cell(const R T10[10], float TAU, R *e_out, R u_out[4]) {
 // ---------------------------------------------------------------------------
 const float kHbarc = 0.197326938;  // src/FreestreamMilne.cpp:141 (float)
 
+// Planted mistakes for the tests that show each pin catches an error in the
+// function it pins (oracle_set_mutation; 0 = none, the only value outside
+// those tests).  Each id changes one term of one function:
+enum Mutation {
+  MUT_NONE = 0,
+  MUT_TRILINEAR_CORNER = 1,  // freeStream 3-D: corners (1,0,0) and (0,1,0) swapped
+  MUT_ETA_NEW_SIGN = 2,      // freeStream 3-D: eta_new = asinh(...) - rap
+  MUT_SHIFT_NO_TAU0 = 3,     // freeStream 3-D: D without the -TAU0 cosh(rap - eta) term
+  MUT_JACOBIAN_ONE_COS = 4,  // stress tensor 3-D: Jacobian with one cos factor
+  MUT_3D_OVER_TAU = 5,       // stress tensor 3-D: acc * d_phip / TAU as in 2-D
+  MUT_CHARGE_WIDTH = 6,      // convertInitialChargeDensity: exp(-d^2 / (2 sigma_B^2))
+  MUT_CHARGE_NORM = 7,       // convertInitialChargeDensity: n without exp(sigma_B^2 / 2)
+  MUT_EDEP_NO_HBARC = 8,     // getEnergyDependentTau: e_T not multiplied by hbarc
+  MUT_EOS2_SWAP = 9,         // calculatePressure EOS 2: a5 and a6 swapped
+  MUT_HYPERTRIG_TAU = 10,    // hypertrig row 3: th instead of th / TAU
+  MUT_ETA_MASK_LOW = 11,     // freeStream 3-D: keeps eta floor index 0
+  MUT_RAP_FACTOR_COSH = 12,  // convertInitialDensity 3-D: cosh instead of cosh^2
+  MUT_BARYON_NO_DRAP = 13,   // calculateBaryonCurrent: J without the DRAP factor
+};
+int g_mutation = MUT_NONE;
+inline bool mut(int m) { return g_mutation == m; }
+
 // EquationOfState.cpp:5-80, one cell.
 template <typename R>
 R pressure_of(R e, int eos) {
   if (eos == 1) return (R)(e / 3.0);
   if (eos == 2) {
-    const R a[13] = {(R)-0.25181736420168666, (R)9737.845799644809, (R)1.077580993288114e6,
+    R a[13] = {(R)-0.25181736420168666, (R)9737.845799644809, (R)1.077580993288114e6,
                      (R)3.1729694865420084e6, (R)1.6357487344679043e6, (R)334334.4309240126,
                      (R)41913.439282708554, (R)6340.448389300905, (R)141.5073484468774,
                      (R)0.7158279081255019, (R)0.0009417586777847889,
@@ -448,6 +470,7 @@ R pressure_of(R e, int eos) {
                      (R)26452.34905933697, (R)499.04919730607065, (R)2.3405487982094204,
                      (R)0.002962497695527404, (R)9.601103399348206e-7,
                      (R)5.928138360995685e-11, (R)3.2581066229887368e-18};
+    if (mut(MUT_EOS2_SWAP)) std::swap(a[5], a[6]);
     R ep[13];
     ep[0] = 1;
     ep[1] = e;
@@ -501,7 +524,7 @@ struct Pipeline {
           R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
           R rap = (R)((double)eta + tan((M_PI / 2.0) * (double)w));
           R d = eta - rap;
-          R rap_factor = (R)((double)(std::cosh(d) * std::cosh(d)) *
+          R rap_factor = (R)((double)(std::cosh(d) * (mut(MUT_RAP_FACTOR_COSH) ? (R)1 : std::cosh(d))) *
                              exp((-1.0) * (double)d * (double)d / (2.0 * (double)SIGMA * (double)SIGMA)));
           dens[(size_t)is * DIM_RAP + irap] = e0[is] * norm_factor * rap_factor;
         }
@@ -516,7 +539,7 @@ struct Pipeline {
     dens.assign((size_t)DIM * DIM_RAP, (R)0);
     // SIGMA_B * SIGMA_B is a float product in the reference (float operands)
     const R sb2 = SIGMA_B * SIGMA_B;
-    R n = (R)(sqrt(2.0 * M_PI) * (double)SIGMA_B * exp((double)sb2 / 2.0));
+    R n = (R)(sqrt(2.0 * M_PI) * (double)SIGMA_B * (mut(MUT_CHARGE_NORM) ? 1.0 : exp((double)sb2 / 2.0)));
     R norm_factor = (R)(1.0 / (2.0 * M_PI * (double)n));
     R rapmin = (R)((-1.0) * ((double)(R)(DIM_RAP - 1) / 2.0) * (double)DRAP);
     R etamin = (R)((-1.0) * ((double)(R)(DIM_ETA - 1) / 2.0) * (double)DETA);
@@ -528,7 +551,8 @@ struct Pipeline {
       for (int irap = 0; irap < DIM_RAP; ++irap) {
         R rap = (R)irap * DRAP + rapmin;
         R d = eta - rap;
-        R rap_factor = (R)((double)std::cosh(d) * exp((-1.0) * (double)d * (double)d / (double)sb2));
+        R rap_factor = (R)((double)std::cosh(d) *
+                           exp((-1.0) * (double)d * (double)d / ((mut(MUT_CHARGE_WIDTH) ? 2.0 : 1.0) * (double)sb2)));
         dens[(size_t)is * DIM_RAP + irap] = n0[is] * norm_factor * rap_factor;
       }
     }
@@ -581,9 +605,10 @@ struct Pipeline {
             x_new = x - std::cos(phip) * DTAU;
             y_new = y - std::sin(phip) * DTAU;
           } else {
-            eta_new = std::asinh((TAU / TAU0) * std::sinh(eta - rap)) + rap;
-            x_new = x - std::cos(phip) * (TAU * std::cosh(rap - eta_new) - TAU0 * std::cosh(rap - eta));
-            y_new = y - std::sin(phip) * (TAU * std::cosh(rap - eta_new) - TAU0 * std::cosh(rap - eta));
+            eta_new = std::asinh((TAU / TAU0) * std::sinh(eta - rap)) + (mut(MUT_ETA_NEW_SIGN) ? -rap : rap);
+            const R t0 = mut(MUT_SHIFT_NO_TAU0) ? (R)0 : TAU0;
+            x_new = x - std::cos(phip) * (TAU * std::cosh(rap - eta_new) - t0 * std::cosh(rap - eta));
+            y_new = y - std::sin(phip) * (TAU * std::cosh(rap - eta_new) - t0 * std::cosh(rap - eta));
           }
           R ix_new = (x_new - xmin) / DX;
           R iy_new = (y_new - ymin) / DY;
@@ -609,14 +634,16 @@ struct Pipeline {
                   (y_min_interp < (double)y_new) && ((double)y_new < y_max_interp))
                 interp_c = (R)spline.eval((double)x_new, (double)y_new);
               interp = (interp_c > 0.0) ? interp_c : interp_l;
-            } else if ((ieta_new_f >= 1) && (ieta_new_f < DIM_ETA - 1)) {
+            } else if ((ieta_new_f >= (mut(MUT_ETA_MASK_LOW) ? 0 : 1)) && (ieta_new_f < DIM_ETA - 1)) {
               int i0 = (int)ix_new_f, j0 = (int)iy_new_f, k0 = (int)ieta_new_f;
               auto D = [&](int a, int b, int c) {
                 return dens[((size_t)(DIM_Y * DIM_ETA * (i0 + a) + DIM_ETA * (j0 + b) + (k0 + c))) * DIM_RAP + irap];
               };
               R x0 = x_frac, x1 = y_frac, x2 = eta_frac;
-              interp = ((1 - x0) * (1 - x1) * (1 - x2) * D(0, 0, 0)) + ((x0) * (1 - x1) * (1 - x2) * D(1, 0, 0)) +
-                       ((1 - x0) * (x1) * (1 - x2) * D(0, 1, 0)) + ((1 - x0) * (1 - x1) * (x2) * D(0, 0, 1)) +
+              const R c100 = mut(MUT_TRILINEAR_CORNER) ? D(0, 1, 0) : D(1, 0, 0);
+              const R c010 = mut(MUT_TRILINEAR_CORNER) ? D(1, 0, 0) : D(0, 1, 0);
+              interp = ((1 - x0) * (1 - x1) * (1 - x2) * D(0, 0, 0)) + ((x0) * (1 - x1) * (1 - x2) * c100) +
+                       ((1 - x0) * (x1) * (1 - x2) * c010) + ((1 - x0) * (1 - x1) * (x2) * D(0, 0, 1)) +
                        ((x0) * (x1) * (1 - x2) * D(1, 1, 0)) + ((x0) * (1 - x1) * (x2) * D(1, 0, 1)) +
                        ((1 - x0) * (x1) * (x2) * D(0, 1, 1)) + ((x0) * (x1) * (x2) * D(1, 1, 1));
             }
@@ -650,7 +677,7 @@ struct Pipeline {
           at(0, irap, iphip, ieta) = 1.0;
           at(1, irap, iphip, ieta) = c / ch;
           at(2, irap, iphip, ieta) = s / ch;
-          at(3, irap, iphip, ieta) = (R)((1.0 / (double)TAU) * (double)th);
+          at(3, irap, iphip, ieta) = (R)((mut(MUT_HYPERTRIG_TAU) ? 1.0 : 1.0 / (double)TAU) * (double)th);
           at(4, irap, iphip, ieta) = (c * c) / (ch * ch);
           at(5, irap, iphip, ieta) = (c * s) / (ch * ch);
           at(6, irap, iphip, ieta) = (R)((1.0 / (double)TAU) * (double)(c * th) / (double)ch);
@@ -677,7 +704,8 @@ struct Pipeline {
         R acc = T[(size_t)ivar * DIM + is];
         for (int irap = 0; irap < DIM_RAP; ++irap) {
           R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
-          R jacobian = (R)((M_PI / 2.0) / cos((M_PI / 2.0) * (double)w) / cos((M_PI / 2.0) * (double)w) *
+          R jacobian = (R)((M_PI / 2.0) / cos((M_PI / 2.0) * (double)w) /
+                           (mut(MUT_JACOBIAN_ONE_COS) ? 1.0 : cos((M_PI / 2.0) * (double)w)) *
                            (1.995 / (double)(R)(DIM_RAP - 1)));
           for (int iphip = 0; iphip < DIM_PHIP; ++iphip) {
             R s = shifted[((size_t)is * DIM_RAP + irap) * DIM_PHIP + iphip];
@@ -688,7 +716,7 @@ struct Pipeline {
               acc += s * h * jacobian;
           }
         }
-        if (DIM_ETA == 1)
+        if (DIM_ETA == 1 || mut(MUT_3D_OVER_TAU))
           acc = acc * d_phip / TAU;
         else
           acc = acc * d_phip;
@@ -715,7 +743,7 @@ struct Pipeline {
           for (int iphip = 0; iphip < DIM_PHIP; ++iphip)
             acc += shifted[((size_t)is * DIM_RAP + irap) * DIM_PHIP + iphip] *
                    H[(((size_t)ivar * DIM_RAP + irap) * DIM_PHIP + iphip) * DIM_ETA + ieta];
-        J[(size_t)ivar * DIM + is] = acc * DRAP * d_phip;
+        J[(size_t)ivar * DIM + is] = acc * (mut(MUT_BARYON_NO_DRAP) ? (R)1 : DRAP) * d_phip;
       }
     }
   }
@@ -889,7 +917,7 @@ int oracle_params_finalize(fsm_params *p, const float *e_in, long n) {
       denominator += eps;
     }
     float e_T = numerator / (denominator + 1.0e-7);
-    e_T = e_T * kHbarc;
+    if (!mut(MUT_EDEP_NO_HBARC)) e_T = e_T * kHbarc;
     float tau_fs = p->TAU_R * pow((e_T / p->E_R), p->ALPHA);
     p->DTAU = tau_fs;
     p->TAU = p->TAU0 + p->DTAU;
@@ -898,6 +926,21 @@ int oracle_params_finalize(fsm_params *p, const float *e_in, long n) {
   int nphip = int(ceil((2.0 * M_PI * p->DTAU) / min_dx_dy));
   p->DIM_PHIP = std::max(p->DIM_PHIP, nphip);
   return 0;
+}
+
+// calculatePressure (src/EquationOfState.cpp:5-80) on n energy densities
+// (fm^-4): precision 0 = float (the reference's fmaf chain), 1 = double.
+void oracle_pressure(int eos, int precision, long n, const double *e, double *p) {
+  for (long i = 0; i < n; ++i)
+    p[i] = precision == 0 ? (double)pressure_of<float>((float)e[i], eos) : pressure_of<double>(e[i], eos);
+}
+
+// Plants mistake `m` (enum Mutation) for the tests of the pins; 0 restores
+// the oracle.  Returns the previous value.
+int oracle_set_mutation(int m) {
+  const int old = g_mutation;
+  g_mutation = m;
+  return old;
 }
 
 // Whole step.  precision: 0 = reference float path, 1 = FP64 oracle.

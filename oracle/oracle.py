@@ -149,6 +149,47 @@ def num_threads() -> int:
     return _lib().oracle_num_threads()
 
 
+def pressure(e, eos: int, precision: str = "float"):
+    """calculatePressure on an array of energy densities (fm^-4)."""
+    lib = _lib()
+    if not hasattr(lib, "_eos_ready"):
+        lib.oracle_pressure.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+        lib.oracle_pressure.restype = None
+        lib._eos_ready = True
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    out = np.empty_like(e)
+    lib.oracle_pressure(eos, 0 if precision == "float" else 1, e.size, _d(e), _d(out))
+    return out
+
+
+class mutation:
+    """Context manager planting one mistake in the oracle (see the Mutation
+    ids in fsm_oracle.cpp), used only by the tests that show a pin fails on
+    it; the oracle is restored on exit."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        lib = _lib()
+        lib.oracle_set_mutation.argtypes = [ctypes.c_int]
+        lib.oracle_set_mutation.restype = ctypes.c_int
+        self.old = lib.oracle_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        _lib().oracle_set_mutation(self.old)
+        return False
+
+
+MUTATIONS = {
+    "trilinear_corner": 1, "eta_new_sign": 2, "shift_no_tau0": 3, "jacobian_one_cos": 4, "3d_over_tau": 5,
+    "charge_width": 6, "charge_norm": 7, "edep_no_hbarc": 8, "eos2_swap": 9, "hypertrig_tau": 10,
+    "eta_mask_low": 11, "rap_factor_cosh": 12, "baryon_no_drap": 13,
+}
+
+
 def landau_condition(T, tau: float, ntrial: int = 4, rel: float = 1e-6, seed: int = 12345, normwise: bool = True):
     """Per-cell condition numbers (kappa_e, kappa_u) of the Landau matching:
     relative change of e (and of u^mu in units of u^tau) per unit relative

@@ -32,6 +32,7 @@ def _case(name):
 def ctx(request):
     c = fsm_amd.Context(0)
     c.set_path(fsm_amd.PATH_MIXED if request.param == "mixed" else fsm_amd.PATH_FP32)
+    c.path_name = request.param
     yield c
     c.close()
 
@@ -43,7 +44,7 @@ def test_baryon_parity(ctx, name):
     outs, nb_g, V_g, q, tau = ctx.run_baryon(p, e, nb)
     ref = O.run(q, e, "float", nB=nb)
     g = np.stack([outs[k] for k in fsm_amd.OUTPUT_NAMES])
-    cond = well_conditioned(ref["T"], q.TAU)
+    cond = well_conditioned(ref["T"], q.TAU, ctx.path_name)
     errs = output_errs(g, ref["out"], np.abs(ref["T"][0]).max(), cond)
     e_ref = ref["out"][0]
     m = (e_ref > 1e-3 * e_ref.max()) & cond

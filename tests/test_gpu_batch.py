@@ -118,3 +118,71 @@ def test_batch_clear_and_reuse(gpu_ctx):
     torch.cuda.synchronize()
     assert np.array_equal(o2.cpu().numpy(), outs[1].cpu().numpy())
     b.close()
+
+
+def test_fp32_batch_of_large_3d_events_matches_single_events():
+    """More 3-D Landau tiles than the batch launch has workgroups (four
+    131^2 x 101 events: 4 x 7511 tiles against 16384 workgroups), so each
+    workgroup runs several tiles in a row, on the FP32 path, whose Landau solve
+    has no barrier of its own.  Every event must equal the same event run
+    alone, bitwise."""
+    import torch
+
+    ctx = fsm_amd.Context(0)
+    ctx.set_path(fsm_amd.PATH_FP32)
+    try:
+        evs = []
+        for seed in (2, 3, 4, 5):
+            p, e = W.c2_like(101, seed=seed)
+            evs.append((fsm_amd.params_finalize(p, e), e))
+        print("FP32 batch of four 131^2x101 events", flush=True)
+        s = torch.cuda.current_stream()
+        ins = [torch.from_numpy(e).cuda() for _, e in evs]
+        singles = []
+        for (q, _), d in zip(evs, ins):
+            o = torch.empty(17 * q.DIM, dtype=torch.float64, device="cuda")
+            ctx.run_device(q, d.data_ptr(), o.data_ptr(), s)
+            torch.cuda.synchronize()
+            singles.append(o.cpu())
+            del o
+        b = fsm_amd.Batch(ctx)
+        outs = [torch.full((17 * q.DIM,), -1.0, dtype=torch.float64, device="cuda") for q, _ in evs]
+        for (q, _), d, o in zip(evs, ins, outs):
+            b.add(q, d.data_ptr(), o.data_ptr())
+        for replay in range(2):
+            b.run(s)
+            torch.cuda.synchronize()
+            for k, (o, ref) in enumerate(zip(outs, singles)):
+                assert torch.equal(o.cpu(), ref), (replay, k)
+        b.close()
+    finally:
+        ctx.close()
+
+
+def test_calls_on_two_streams_are_ordered(gpu_ctx):
+    """Two same-shape events on two streams of one context share the
+    context's workspace; the second call waits for the first, so each result
+    equals the event run alone."""
+    import torch
+
+    pa, ea = W.config("C1")
+    pb, eb = W.config("C1", seed=9)
+    qa, qb = fsm_amd.params_finalize(pa, ea), fsm_amd.params_finalize(pb, eb)
+    s0 = torch.cuda.current_stream()
+    da, db = torch.from_numpy(ea).cuda(), torch.from_numpy(eb).cuda()
+    refs = []
+    for q, d in ((qa, da), (qb, db)):
+        o = torch.empty(17 * q.DIM, dtype=torch.float64, device="cuda")
+        gpu_ctx.run_device(q, d.data_ptr(), o.data_ptr(), s0)
+        torch.cuda.synchronize()
+        refs.append(o.cpu())
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for rep in range(3):
+        oa = torch.empty(17 * qa.DIM, dtype=torch.float64, device="cuda")
+        ob = torch.empty(17 * qb.DIM, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(4):
+            gpu_ctx.run_device(qa, da.data_ptr(), oa.data_ptr(), s1)
+            gpu_ctx.run_device(qb, db.data_ptr(), ob.data_ptr(), s2)
+        torch.cuda.synchronize()
+        assert torch.equal(oa.cpu(), refs[0]) and torch.equal(ob.cpu(), refs[1]), rep

@@ -64,7 +64,7 @@ def test_every_output_against_fp64(name):
     # (+2) cells from the transverse edges -- in 3-D too, where the central
     # rapidity node (w = 0, eta_new = eta) shifts by exactly DTAU
     inner = ring_interior_mask(q)
-    keep = inner & (dref["out"][0] > 1e-3 * dref["out"][0].max()) & well_conditioned(dref["T"], q.TAU)
+    keep = inner & (dref["out"][0] > 1e-3 * dref["out"][0].max()) & well_conditioned(dref["T"], q.TAU, "mixed")
     rows = {"reference float path": (fref["T"], fref["out"])}
     for label, path in (("GPU FP32 path", fsm_amd.PATH_FP32), ("GPU mixed path", fsm_amd.PATH_MIXED)):
         ctx = fsm_amd.Context(0)

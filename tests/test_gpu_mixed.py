@@ -13,7 +13,7 @@ import pytest
 import fsm_amd
 import fsm_amd.workloads as W
 import oracle as O
-from parity import TOL, interior_mask, output_errs, ring_interior_mask, tmunu_err, well_conditioned
+from parity import TOL, excluded_cells_check, interior_mask, output_errs, ring_interior_mask, tmunu_err, well_conditioned
 
 pytestmark = pytest.mark.gpu
 
@@ -92,13 +92,17 @@ def test_mixed_parity(mixed_ctx, name):
     inner = ring_interior_mask(q) if q.DIM_ETA == 1 else interior_mask(q)
     td = tmunu_err(T, dref["T"], keep=inner)
     T00max = np.abs(ref["T"][0]).max()
-    cond = well_conditioned(ref["T"], q.TAU)
+    cond = well_conditioned(ref["T"], q.TAU, "mixed")
     errs = output_errs(out, ref["out"], T00max, cond)
     errs64 = output_errs(out, dref["out"], np.abs(dref["T"][0]).max(), cond & inner)
     print(name, "mixed T err vs float path %.3g, vs FP64 (cells whose samples stay inside) %.3g" % (tf, td),
           flush=True)
     print("  vs float path", errs, flush=True)
     print("  vs FP64      ", errs64, flush=True)
+    # rect3d streams over a ring wider than the grid: most cells see one side
+    exc = excluded_cells_check(out, ref["out"], dref["out"], cond, q, "mixed", ref["T"], dref["T"],
+                               budget=0.75 if name == "rect3d" else None)
+    print("  cells left out of the e/u bar, against FP64:", exc, flush=True)
     assert tf <= TOL
     if q.DIM_ETA == 1:
         # In 3-D the float path itself differs from FP64 where a sample sits on
@@ -119,7 +123,7 @@ def test_mixed_parity_c2_full(mixed_ctx):
     T, out = _gpu(mixed_ctx, q, e)
     ref = O.run(q, e, "float")
     tf = tmunu_err(T, ref["T"])
-    errs = output_errs(out, ref["out"], np.abs(ref["T"][0]).max(), well_conditioned(ref["T"], q.TAU))
+    errs = output_errs(out, ref["out"], np.abs(ref["T"][0]).max(), well_conditioned(ref["T"], q.TAU, "mixed"))
     print("C2 mixed T err vs float path %.3g" % tf, errs, flush=True)
     assert tf <= TOL
     for k in ("e", "p", "u", "pi", "Pi"):

@@ -10,7 +10,7 @@ import pytest
 import fsm_amd
 import fsm_amd.workloads as W
 import oracle as O
-from parity import TOL, interior_mask, output_errs, tmunu_err, well_conditioned
+from parity import TOL, excluded_cells_check, interior_mask, output_errs, tmunu_err, well_conditioned
 
 pytestmark = pytest.mark.gpu
 
@@ -72,8 +72,11 @@ def test_parity_against_reference_float_path(gpu_ctx, name):
     tf = tmunu_err(T, ref["T"])
     td = tmunu_err(T, dref["T"], keep=interior_mask(q))
     T00max = np.abs(ref["T"][0]).max()
-    errs = output_errs(out, ref["out"], T00max, well_conditioned(ref["T"], q.TAU))
+    cond = well_conditioned(ref["T"], q.TAU)
+    errs = output_errs(out, ref["out"], T00max, cond)
     print(name, "T err vs float path %.3g, vs FP64 (interior) %.3g" % (tf, td), errs, flush=True)
+    exc = excluded_cells_check(out, ref["out"], dref["out"], cond, q, "fp32", ref["T"], dref["T"])
+    print(name, "ill-conditioned cells against FP64:", exc, flush=True)
     assert tf <= TOL
     for k in ("e", "p", "u", "pi", "Pi"):
         assert errs[k] <= TOL, (k, errs[k])
@@ -86,8 +89,12 @@ def test_parity_c2_full(gpu_ctx):
     T, out = _gpu(gpu_ctx, q, e)
     ref = O.run(q, e, "float")
     tf = tmunu_err(T, ref["T"])
-    errs = output_errs(out, ref["out"], np.abs(ref["T"][0]).max(), well_conditioned(ref["T"], q.TAU))
+    cond = well_conditioned(ref["T"], q.TAU)
+    errs = output_errs(out, ref["out"], np.abs(ref["T"][0]).max(), cond)
     print("C2 T err vs float path %.3g" % tf, errs, flush=True)
+    dref = O.run(q, e, "double", want_extra=False)
+    print("C2 ill-conditioned cells against FP64:",
+          excluded_cells_check(out, ref["out"], dref["out"], cond, q, "fp32", ref["T"], dref["T"]), flush=True)
     assert tf <= TOL
     for k in ("e", "p", "u", "pi", "Pi"):
         assert errs[k] <= TOL, (k, errs[k])
@@ -137,8 +144,10 @@ def test_switches_match_reference(gpu_ctx, variant):
         return
     T00max = np.abs(ref["T"][0]).max()
     errs = output_errs(out, ref["out"], T00max, well_conditioned(ref["T"], q.TAU))
-    for k in ("e", "u", "pi", "Pi"):
+    for k in ("e", "p", "u", "pi", "Pi"):
         assert errs[k] <= TOL, (variant, k, errs[k])
+    if variant == "eos3":
+        assert np.all(out[1] == 0.0)  # the placeholder EoS gives p = 0
     if variant == "ideal":
         assert np.all(out[6:] == 0.0)
 
