@@ -211,6 +211,7 @@ struct fsm_ctx {
   // host-API workspace
   DevBuf<float> d_in, d_nb, d_T, d_J;
   DevBuf<double> d_out;
+  DevBuf<float> d_stage;  // planes of the stage entry points on host arrays
   // One-event calls share the workspace above and the plan's event-sized
   // buffers (spline planes, MFMA planes, 3-D source planes, eta-major T), so
   // they are ordered: a call's stream first waits for the previous call's
@@ -313,11 +314,11 @@ void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char>
   }
   const size_t no = t.offs.size();
   // B operand of one MFMA from its 32x6 weights wt(g, c, m) (corner/lane
-  // group g, knot component c, moment m): columns 0-5 hold W_h for both the
-  // hi and lo knot halves, columns 8-13 W_l for the hi half.  Stored packed:
-  // lane l's four distinct halves (W_h or W_l of its column for c = 0..3),
-  // 8 bytes, 64 lanes in 32 uint4; the kernel repeats them in the upper
-  // halves for columns 0-5 (fsm_stream2d.hip, bmat_expand).
+  // group g, knot component c, moment m): columns 0-5 hold W_h, columns 8-13
+  // W_l, both for the hi and the lo knot halves.  Stored packed: lane l's
+  // four distinct halves (W_h or W_l of its column for c = 0..3), 8 bytes, 64
+  // lanes in 32 uint4; the kernel repeats them in the upper halves
+  // (fsm_stream2d.hip, bmat_expand).
   auto bmat_lanes = [](auto wt, uint4 *out) {
     unsigned packed[128];
     for (int lane = 0; lane < 64; ++lane) {
@@ -929,6 +930,31 @@ int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *ou
   return FSM_OK;
 }
 
+// Batched device time per unit of work on MI355X, in picoseconds, from the
+// measured stage times of the 261^2 lumpy event (64-event batches: stream
+// 14.4 / 8.5 us for 13.6 M samples, spline 1.5 us and Landau 3.0 us for 68 k
+// cells) and of the 131^2 x 101 event (stream 0.76 ms for 6.8e8 evaluated
+// samples, Landau 0.14 ms for 1.73 M cells): 2-D stream per (cell, phi)
+// sample on each path, 3-D stream per evaluated sample, spline or source
+// planes and Landau per cell.
+const double kCostStream2dPs[2] = {1.06, 0.62};  // FP32, mixed
+const double kCostStream3dPs = 1.12;
+const double kCostPlanes2dPs = 22.0, kCostPlanes3dPs = 17.0;
+const double kCostLandau2dPs = 44.0, kCostLandau3dPs = 81.0;
+
+// Estimated device time of one event inside a batch, microseconds.
+double event_cost_us(const Plan &pl, const fsm_params &p, int path, bool baryon) {
+  const int passes = baryon ? 2 : 1;  // energy and charge streaming
+  const double D = (double)p.DIM;
+  double ps;
+  if (pl.is3d)
+    ps = passes * (pl.samples_evaluated * kCostStream3dPs + D * kCostPlanes3dPs) + D * kCostLandau3dPs;
+  else
+    ps = passes * (pl.samples_evaluated * kCostStream2dPs[path == FSM_PATH_MIXED ? 1 : 0] + D * kCostPlanes2dPs) +
+         D * kCostLandau2dPs;
+  return ps * 1e-6;
+}
+
 int get_plan(fsm_ctx *ctx, const fsm_params &p, Plan **out) {
   std::shared_ptr<Plan> sp;
   int st = get_plan_shared(ctx, p, &sp);
@@ -1211,6 +1237,7 @@ int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info) {
   info->dim3 = pl->is3d ? 1 : 0;
   info->samples_kept = pl->samples_kept;
   info->samples_evaluated = pl->samples_evaluated;
+  info->est_batch_us = (float)event_cost_us(*pl, *p, ctx->path, false);
   if (pl->is3d) {
     info->tiles = pl->ev3.tiles * pl->ev3.neta;
     info->n_phi = pl->ev3.nphi;
@@ -1356,6 +1383,151 @@ int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T, float
   tmark(ctx, 2, s);
   FSM_TRY(fsm_launch_landau(L, s));
   tmark(ctx, 3, s);
+  return FSM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// The stage entry points on the reference's host arrays (planes of DIM
+// floats, one pointer per component): copy in, one kernel, copy out, on the
+// null stream, holding the context's call lock.
+struct StageIO {
+  fsm_ctx *ctx;
+  size_t D;
+  float *base = nullptr;
+  hipStream_t s = nullptr;
+  int put(const float *const *planes, int n, size_t at) {
+    for (int c = 0; c < n; ++c) {
+      if (!planes[c]) return FSM_ERR_ARG;
+      FSM_TRY(hipMemcpyAsync(base + (at + c) * D, planes[c], D * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    return FSM_OK;
+  }
+  int get(float *const *planes, int n, size_t at) {
+    for (int c = 0; c < n; ++c) {
+      if (!planes[c]) return FSM_ERR_ARG;
+      FSM_TRY(hipMemcpyAsync(planes[c], base + (at + c) * D, D * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    return FSM_OK;
+  }
+};
+
+int stage_begin(fsm_ctx *ctx, const fsm_params *p, size_t planes, StageIO *io) {
+  if (!ctx || !p || p->DIM <= 0 || p->DIM != p->DIM_X * p->DIM_Y * p->DIM_ETA) return FSM_ERR_ARG;
+  FSM_TRY(hipSetDevice(ctx->device));
+  io->ctx = ctx;
+  io->D = (size_t)p->DIM;
+  FSM_TRY(ctx->d_stage.ensure(planes * io->D));
+  io->base = ctx->d_stage.p;
+  return FSM_OK;
+}
+
+StageCells stage_args(const fsm_params &p, int stage) {
+  StageCells a;
+  memset(&a, 0, sizeof(a));
+  a.dim = p.DIM;
+  a.stage = stage;
+  a.eos_type = p.EOS_TYPE;
+  a.tau = p.TAU;
+  const float t2 = p.TAU * p.TAU;  // a float product, as the reference forms it
+  a.inv_tau2 = 1.0 / (double)t2;
+  return a;
+}
+}  // namespace
+
+extern "C" {
+
+int fsm_solve_eigen_system(fsm_ctx *ctx, const fsm_params *p, const float *const T[10], float *e,
+                           float *const u[4]) {
+  if (!T || !e || !u) return FSM_ERR_ARG;
+  if (!ctx) return FSM_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);  // ctx->d_stage
+  StageIO io;
+  int st = stage_begin(ctx, p, 15, &io);  // T (10), e, u (4)
+  if (st) return st;
+  CallOrder order(ctx, io.s);
+  FSM_TRY(order.begin());
+  if ((st = io.put(T, 10, 0))) return st;
+  EventLandau L = landau_event(*p, io.base, 0, ctx->path == FSM_PATH_MIXED);
+  L.fe = io.base + 10 * io.D;
+  L.fu = io.base + 11 * io.D;
+  FSM_TRY(fsm_launch_landau(L, io.s));
+  float *ep[1] = {e};
+  if ((st = io.get(ep, 1, 10)) || (st = io.get(u, 4, 11))) return st;
+  FSM_TRY(hipStreamSynchronize(io.s));
+  return FSM_OK;
+}
+
+int fsm_calculate_pressure(fsm_ctx *ctx, const fsm_params *p, const float *e, const float *nB, float *pr) {
+  (void)nB;  // none of the reference's equations of state reads it (src/EquationOfState.cpp:5-80)
+  if (!e || !pr) return FSM_ERR_ARG;
+  if (!ctx) return FSM_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);  // ctx->d_stage
+  StageIO io;
+  int st = stage_begin(ctx, p, 2, &io);
+  if (st) return st;
+  CallOrder order(ctx, io.s);
+  FSM_TRY(order.begin());
+  const float *ein[1] = {e};
+  if ((st = io.put(ein, 1, 0))) return st;
+  StageCells a = stage_args(*p, FSM_STAGE_PRESSURE);
+  a.e = io.base;
+  a.p_out = io.base + io.D;
+  FSM_TRY(fsm_launch_stage_cells(a, io.s));
+  float *pout[1] = {pr};
+  if ((st = io.get(pout, 1, 1))) return st;
+  FSM_TRY(hipStreamSynchronize(io.s));
+  return FSM_OK;
+}
+
+int fsm_calculate_bulk_pressure(fsm_ctx *ctx, const fsm_params *p, const float *const T[10], const float *e,
+                                const float *pr, float *Pi) {
+  if (!T || !e || !pr || !Pi) return FSM_ERR_ARG;
+  if (!ctx) return FSM_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);  // ctx->d_stage
+  StageIO io;
+  int st = stage_begin(ctx, p, 13, &io);  // T (10), e, p, Pi
+  if (st) return st;
+  CallOrder order(ctx, io.s);
+  FSM_TRY(order.begin());
+  const float *in2[2] = {e, pr};
+  if ((st = io.put(T, 10, 0)) || (st = io.put(in2, 2, 10))) return st;
+  StageCells a = stage_args(*p, FSM_STAGE_BULK);
+  a.T = io.base;
+  a.e = io.base + 10 * io.D;
+  a.p = io.base + 11 * io.D;
+  a.Pi_out = io.base + 12 * io.D;
+  FSM_TRY(fsm_launch_stage_cells(a, io.s));
+  float *out[1] = {Pi};
+  if ((st = io.get(out, 1, 12))) return st;
+  FSM_TRY(hipStreamSynchronize(io.s));
+  return FSM_OK;
+}
+
+int fsm_calculate_shear_visc_tensor(fsm_ctx *ctx, const fsm_params *p, const float *const T[10], const float *e,
+                                    const float *const u[4], const float *pr, const float *Pi, float *const pi[10]) {
+  if (!T || !e || !u || !pr || !Pi || !pi) return FSM_ERR_ARG;
+  if (!ctx) return FSM_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);  // ctx->d_stage
+  StageIO io;
+  int st = stage_begin(ctx, p, 27, &io);  // T (10), e, u (4), p, Pi, pi (10)
+  if (st) return st;
+  CallOrder order(ctx, io.s);
+  FSM_TRY(order.begin());
+  const float *ein[1] = {e}, *pin[2] = {pr, Pi};
+  if ((st = io.put(T, 10, 0)) || (st = io.put(ein, 1, 10)) || (st = io.put(u, 4, 11)) || (st = io.put(pin, 2, 15)))
+    return st;
+  StageCells a = stage_args(*p, FSM_STAGE_SHEAR);
+  a.T = io.base;
+  a.e = io.base + 10 * io.D;
+  a.u = io.base + 11 * io.D;
+  a.p = io.base + 15 * io.D;
+  a.Pi = io.base + 16 * io.D;
+  a.pi_out = io.base + 17 * io.D;
+  FSM_TRY(fsm_launch_stage_cells(a, io.s));
+  if ((st = io.get(pi, 10, 17))) return st;
+  FSM_TRY(hipStreamSynchronize(io.s));
   return FSM_OK;
 }
 
@@ -2121,32 +2293,13 @@ int batch_launch_now(fsm_batch *b, hipStream_t s) {
 
 int ctx_device(const fsm_ctx *ctx) { return ctx->device; }
 
-// Batched device time per unit of work on MI355X, from the measured stage
-// times of the 261^2 lumpy event (64-event batches) and of the 131^2 x 101
-// event: 2-D stream per (cell, phi) sample on each path, 3-D stream per
-// evaluated sample, spline / source planes and Landau per cell.
-const float kCostStream2dNs[2] = {1.06f, 0.62f};  // FP32, mixed
-const float kCostStream3dNs = 1.10f;
-const float kCostPlanes2dNs = 0.022f, kCostPlanes3dNs = 0.010f;
-const float kCostLandau2dNs = 0.044f, kCostLandau3dNs = 0.081f;
-
 int plan_check(fsm_ctx *ctx, const fsm_params *p, bool baryon, float *cost_us) {
   std::shared_ptr<Plan> sp;
   int st = get_plan_shared(ctx, *p, &sp);
   if (st) return st;
   st = maybe_tune(ctx, *sp, *p);
   if (st) return st;
-  if (cost_us) {
-    const int passes = baryon ? 2 : 1;  // energy and charge streaming
-    const double D = (double)p->DIM;
-    double ns;
-    if (sp->is3d)
-      ns = passes * (sp->samples_evaluated * kCostStream3dNs + D * kCostPlanes3dNs) + D * kCostLandau3dNs;
-    else
-      ns = passes * (sp->samples_evaluated * kCostStream2dNs[ctx->path == FSM_PATH_MIXED ? 1 : 0] +
-                     D * kCostPlanes2dNs) + D * kCostLandau2dNs;
-    *cost_us = (float)(ns * 1e-3);
-  }
+  if (cost_us) *cost_us = (float)event_cost_us(*sp, *p, ctx->path, baryon);
   return FSM_OK;
 }
 

@@ -132,6 +132,23 @@ struct StencilGroup {
   int pad0, pad1;
 };
 
+// Arguments of k_stage_cells: one of the reference's per-cell stages on
+// planes of `dim` floats in the reference layout.
+enum { FSM_STAGE_PRESSURE = 0, FSM_STAGE_BULK = 1, FSM_STAGE_SHEAR = 2 };
+struct StageCells {
+  long dim;
+  int stage;
+  int eos_type;
+  float tau;
+  double inv_tau2;      // 1.0 / (double)(TAU * TAU), TAU * TAU a float product
+  const float *T;       // 10 planes (bulk, shear)
+  const float *e;       // energy density
+  const float *u;       // 4 planes (shear)
+  const float *p;       // pressure (bulk, shear)
+  const float *Pi;      // bulk pressure (shear)
+  float *p_out, *Pi_out, *pi_out;  // pi_out: 10 planes
+};
+
 struct Event2D {
   int nx, ny;
   int ngroups;
@@ -185,7 +202,7 @@ struct Event2D {
 //   T_m(cell) = sum_o sum_{corner,comp} K[cell+o+corner][comp] * W_o[corner,comp][m],
 //   W_o = sum_{j in o} hermite_j[corner,comp] * P_j[m],
 // one 16x16x32 FP16 MFMA per 16 cells and offset, with the knots and the
-// weights split into hi + lo halves (K_h W_h + K_l W_h + K_h W_l).  Knots are
+// weights split into hi + lo halves ((K_h + K_l)(W_h + W_l), all four products).  Knots are
 // multiplied by a per-event power of two before the split, chosen from the
 // event's largest |z| so that the largest knot value is near
 // FSM_MFMA_ZTARGET (the fp16 subnormal floor then sits ~2^-36 below it; the

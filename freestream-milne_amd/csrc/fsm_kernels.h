@@ -14,6 +14,8 @@ hipError_t fsm_launch_stream3d(const Event3D &ev, hipStream_t s);
 hipError_t fsm_launch_transpose(const float *in, float *out, int rows, int cols, int planes, long in_plane,
                                 long out_plane, hipStream_t s);
 hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s);
+// One per-cell stage (pressure, bulk pressure or shear tensor) on given planes.
+hipError_t fsm_launch_stage_cells(const StageCells &a, hipStream_t s);
 // Raises the dynamic-LDS ceiling of the stream kernels (bytes per workgroup).
 hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d);
 

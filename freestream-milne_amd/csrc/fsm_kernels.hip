@@ -1000,6 +1000,33 @@ __device__ float eos_pressure(float e, int eos) {
   return 0.0f;
 }
 
+// calculateBulkPressure (src/LandauMatch.cpp:332-344), one cell, in the
+// reference's float expressions (contraction off).
+__device__ __forceinline__ float bulk_pressure(const float t[10], float TAU, float e, float p) {
+#pragma clang fp contract(off)
+  const float a = t[0] - t[4] - t[7] - TAU * TAU * t[9];
+  return (float)((-1.0 / 3.0) * (double)(a - e) - (double)p);
+}
+
+// calculateShearViscTensor (src/LandauMatch.cpp:345-367), one cell;
+// inv_tau2 = 1.0 / (double)(TAU * TAU) with TAU * TAU a float product.
+__device__ __forceinline__ void shear_tensor(const float t[10], double inv_tau2, float e, const float u[4], float p,
+                                             float Pi, float pi[10]) {
+#pragma clang fp contract(off)
+  const float b = e + p + Pi;
+  const float c = p + Pi;
+  pi[0] = t[0] - u[0] * u[0] * b + c;
+  pi[1] = t[1] - u[0] * u[1] * b;
+  pi[2] = t[2] - u[0] * u[2] * b;
+  pi[3] = t[3] - u[0] * u[3] * b;
+  pi[4] = t[4] - u[1] * u[1] * b - c;
+  pi[5] = t[5] - u[1] * u[2] * b;
+  pi[6] = t[6] - u[1] * u[3] * b;
+  pi[7] = t[7] - u[2] * u[2] * b - c;
+  pi[8] = t[8] - u[2] * u[3] * b;
+  pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * inv_tau2);
+}
+
 // Results of one cell in the reference's float precision.
 struct CellOut {
   float e, p, u[4], pi[10], Pi;
@@ -1092,21 +1119,8 @@ __device__ void landau_cell_compute(const EventLandau &ev, bool live, long tidx,
 #pragma unroll
   for (int c = 0; c < 10; ++c) r.pi[c] = 0.0f;
   if (ev.viscous) {
-    // src/LandauMatch.cpp:332-367
-    const float a = t[0] - t[4] - t[7] - TAU * TAU * t[9];
-    Pi = (float)((-1.0 / 3.0) * (double)(a - e) - (double)p);
-    const float b = e + p + Pi;
-    const float c = p + Pi;
-    r.pi[0] = t[0] - u[0] * u[0] * b + c;
-    r.pi[1] = t[1] - u[0] * u[1] * b;
-    r.pi[2] = t[2] - u[0] * u[2] * b;
-    r.pi[3] = t[3] - u[0] * u[3] * b;
-    r.pi[4] = t[4] - u[1] * u[1] * b - c;
-    r.pi[5] = t[5] - u[1] * u[2] * b;
-    r.pi[6] = t[6] - u[1] * u[3] * b;
-    r.pi[7] = t[7] - u[2] * u[2] * b - c;
-    r.pi[8] = t[8] - u[2] * u[3] * b;
-    r.pi[9] = (float)((double)(t[9] - u[3] * u[3] * b) - (double)c * ev.inv_tau2);
+    Pi = bulk_pressure(t, TAU, e, p);
+    shear_tensor(t, ev.inv_tau2, e, u, p, Pi, r.pi);
   }
   r.e = e;
   r.p = p;
@@ -1223,6 +1237,32 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds,
   __syncthreads();  // every lane has read its staged outputs
 }
 
+// One stage of the reference's per-cell chain on caller-given inputs
+// (calculatePressure, calculateBulkPressure or calculateShearViscTensor):
+// one cell per thread over planes of `dim` floats.
+__global__ __launch_bounds__(256) void k_stage_cells(StageCells a) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long is = (long)blockIdx.x * blockDim.x + threadIdx.x; is < a.dim; is += stride) {
+    if (a.stage == FSM_STAGE_PRESSURE) {
+      a.p_out[is] = eos_pressure(a.e[is], a.eos_type);
+      continue;
+    }
+    float t[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) t[c] = a.T[c * a.dim + is];
+    if (a.stage == FSM_STAGE_BULK) {
+      a.Pi_out[is] = bulk_pressure(t, a.tau, a.e[is], a.p[is]);
+      continue;
+    }
+    float u[4], pi[10];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) u[c] = a.u[c * a.dim + is];
+    shear_tensor(t, a.inv_tau2, a.e[is], u, a.p[is], a.Pi[is], pi);
+#pragma unroll
+    for (int c = 0; c < 10; ++c) a.pi_out[c * a.dim + is] = pi[c];
+  }
+}
+
 template <int N>
 __global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
   __shared__ LandauShared S;
@@ -1300,6 +1340,13 @@ hipError_t fsm_launch_src3d(const Event3D &ev, hipStream_t s) {
 hipError_t fsm_launch_stream3d(const Event3D &ev, hipStream_t s) {
   size_t lds = (size_t)ev.lds_floats * sizeof(float);
   hipLaunchKernelGGL(k_stream3d_one, dim3(ev.tiles * ev.neta), dim3(FSM3D_THREADS), lds, s, ev);
+  return hipGetLastError();
+}
+
+hipError_t fsm_launch_stage_cells(const StageCells &a, hipStream_t s) {
+  if (a.dim <= 0) return hipSuccess;
+  const int blocks = (int)std::min<long>((a.dim + 255) / 256, 16384L);
+  hipLaunchKernelGGL(k_stage_cells, dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -78,6 +78,36 @@ __device__ __forceinline__ void stage_copy(T *dst, const T *__restrict__ src, in
   }
 }
 
+// The reference's value of a kept sample whose source point is not strictly
+// inside the spline's knot range, where it uses bilinear interpolation with
+// its own float fractions (src/FreeStream.cpp:198-246; the range test compares
+// the float source point, promoted to double, with the grid coordinates).
+// For a kept sample that only happens on the last knot line: a shift smaller
+// than half an ulp of the coordinate leaves x_new == x_max, and the rounded
+// (x_new - xmin)/DX can fall just below n - 1, so the reference keeps the
+// sample at floor n - 2 with a fraction near 1.  Returns false (nothing
+// written) when the point is inside the range.  Call only for kept samples.
+__device__ __forceinline__ bool ref_off_spline(const Event2D &ev, const Stencil2D &S, int ix, int iy, float *out) {
+#pragma clang fp contract(off)
+  const float x = (float)ix * ev.dx + ev.xmin, y = (float)iy * ev.dy + ev.ymin;
+  const float xn = x - S.dxj, yn = y - S.dyj;
+  const double xmax = (double)((float)(ev.nx - 1) * ev.dx + ev.xmin);
+  const double ymax = (double)((float)(ev.ny - 1) * ev.dy + ev.ymin);
+  if ((double)ev.xmin < (double)xn && (double)xn < xmax && (double)ev.ymin < (double)yn && (double)yn < ymax)
+    return false;
+  const float ixn = (xn - ev.xmin) / ev.dx, iyn = (yn - ev.ymin) / ev.dy;
+  const float fx = floorf(ixn), fy = floorf(iyn);
+  if (!(fx >= 1.0f && fx < (float)(ev.nx - 1) && fy >= 1.0f && fy < (float)(ev.ny - 1))) return false;
+  const float x0 = ixn - fx, x1 = iyn - fy;
+  const int i0 = (int)fx, j0 = (int)fy;
+  const float a00 = fsm_gload(ev.planes + (long)i0 * ev.ny + j0).x;
+  const float a10 = fsm_gload(ev.planes + (long)(i0 + 1) * ev.ny + j0).x;
+  const float a01 = fsm_gload(ev.planes + (long)i0 * ev.ny + j0 + 1).x;
+  const float a11 = fsm_gload(ev.planes + (long)(i0 + 1) * ev.ny + j0 + 1).x;
+  *out = ((1 - x0) * (1 - x1) * a00) + ((x0) * (1 - x1) * a10) + ((1 - x0) * (x1) * a01) + ((x0) * (x1) * a11);
+  return true;
+}
+
 template <int M, int JS>
 __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, float4 *lds) {
   constexpr int CG = 4 / JS;          // cell groups per workgroup
@@ -159,10 +189,13 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
       }
       if (!interior) {
         const bool yok = fsm_mask_ok(iy, iy + oy, S.dyj, ev.ymin, ev.dy, ny);
+        const bool ymax_line = iy + oy >= ny - 2;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           const bool ok = yok && fsm_mask_ok(ixb + m, ixb + m + ox, S.dxj, ev.xmin, ev.dx, nx);
           v[m] = ok ? v[m] : 0.0f;
+          // a kept sample on the last knot line: the reference's bilinear
+          if (ok && (ymax_line || ixb + m + ox >= nx - 2)) (void)ref_off_spline(ev, S, ixb + m, iy, &v[m]);
         }
       }
 #pragma unroll
@@ -239,8 +272,8 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
 // B[8*(l/16)+0..7][l%16], and D[4*(l/16)+0..3][l%16].  Rows of A are 16 cells along y,
 // the K index is 8*corner + (hi z, zy, zx, zxy, lo z, zy, zx, zxy), so lane
 // (corner, cell) reads its knot's 16-byte record from LDS as one load; columns
-// 0-5 of B carry W_h for both halves and columns 8-13 carry W_l for the hi half,
-// so D[:, m] + D[:, m+8] = K_h W_h + K_l W_h + K_h W_l for moment m
+// 0-5 of B carry W_h and columns 8-13 W_l, each for both knot halves, so
+// D[:, m] + D[:, m+8] = (K_h + K_l)(W_h + W_l) for moment m
 // (1, c, s, c^2, cs, s^2).
 //
 // Wave w owns tile columns [w*XW, (w+1)*XW) x 32 rows (two 16-cell blocks).
@@ -285,20 +318,21 @@ __device__ __forceinline__ float sample_correction(const Event2D &ev, const Sten
     const float b0 = fmaf(S.ty, k01.x - k00.x, k00.x), b1 = fmaf(S.ty, k11.x - k10.x, k10.x);
     exact = fmaf(S.tx, b1 - b0, b0);
   }
+  if (ok) (void)ref_off_spline(ev, S, ix, iy, &exact);
   return (ok ? exact : 0.0f) - (idx ? v : 0.0f);
 }
 
-// B operands are stored packed: lane l's four distinct halves (8 bytes).
-// Columns 0-5 (lane % 16 < 6) repeat them in halves 4-7, the W_h of both knot
-// halves; the other columns keep halves 4-7 zero.  `b` is the operand for
-// cell columns sharing the lane's corner roles, `bs` for the odd columns
-// (B'[lane] = B[lane ^ 16]: ds_swizzle bit mode and 0x1f, xor 0x10).
-__device__ __forceinline__ void bmat_expand(const uint4 *bl, int m, int lane, bool dup, fsm_half8 &b, fsm_half8 &bs) {
+// B operands are stored packed: lane l's four distinct halves (8 bytes, W_h
+// in columns 0-5, W_l in columns 8-13, zero in the others), repeated in
+// halves 4-7 so that both knot halves meet both weight halves.  `b` is the
+// operand for cell columns sharing the lane's corner roles, `bs` for the odd
+// columns (B'[lane] = B[lane ^ 16]: ds_swizzle bit mode and 0x1f, xor 0x10).
+__device__ __forceinline__ void bmat_expand(const uint4 *bl, int m, int lane, fsm_half8 &b, fsm_half8 &bs) {
   const uint2 v = reinterpret_cast<const uint2 *>(bl + m * FSM_BMAT_U4)[lane];
   const uint2 w = make_uint2((unsigned)__builtin_amdgcn_ds_swizzle((int)v.x, 0x401F),
                              (unsigned)__builtin_amdgcn_ds_swizzle((int)v.y, 0x401F));
-  b = __builtin_bit_cast(fsm_half8, make_uint4(v.x, v.y, dup ? v.x : 0u, dup ? v.y : 0u));
-  bs = __builtin_bit_cast(fsm_half8, make_uint4(w.x, w.y, dup ? w.x : 0u, dup ? w.y : 0u));
+  b = __builtin_bit_cast(fsm_half8, make_uint4(v.x, v.y, v.x, v.y));
+  bs = __builtin_bit_cast(fsm_half8, make_uint4(w.x, w.y, w.x, w.y));
 }
 
 // The MFMA loop over the knot offsets of one LDS stage, for the wave's XW
@@ -319,13 +353,12 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
                                              int xw0, int y0, int nx, int ny) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
-  const bool dup = (lane & 15) < 6;
   for (int o = 0; o < no; ++o) {
     const int4 rec = orec[o];
     const int ox = __builtin_amdgcn_readfirstlane(rec.x), oy = __builtin_amdgcn_readfirstlane(rec.y);
     const int delta = __builtin_amdgcn_readfirstlane(rec.z);
     fsm_half8 b0, b1;
-    bmat_expand(bl, o, lane, dup, b0, b1);
+    bmat_expand(bl, o, lane, b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
       // one 16-row block at a time: the knot registers of a block are
@@ -382,12 +415,11 @@ __device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint
                                            const int4 *prec, int np, int lane_base, int LYP, int gx, int gy) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
-  const bool dup = (lane & 15) < 6;
   for (int o = 0; o < np; ++o) {
     const int4 rec = prec[o];
     const int delta = gy ? rec.y : rec.x;
     fsm_half8 b0, b1;
-    bmat_expand(bl, o, lane, dup, b0, b1);
+    bmat_expand(bl, o, lane, b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
       uint4 R[NR];

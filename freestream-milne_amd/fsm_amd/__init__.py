@@ -103,7 +103,7 @@ class PlanInfo(ctypes.Structure):
                                             "n_offset_groups", "n_vector_samples", "lds_bytes", "batch_variant",
                                             "batch_tiles", "tuned")] + [("tune_us", ctypes.c_float * 3)] + \
         [(n, ctypes.c_int) for n in ("mfma_batch", "mfma_one")] + \
-        [(n, ctypes.c_double) for n in ("samples_kept", "samples_evaluated")]
+        [(n, ctypes.c_double) for n in ("samples_kept", "samples_evaluated")] + [("est_batch_us", ctypes.c_float)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

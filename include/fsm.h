@@ -153,6 +153,8 @@ typedef struct fsm_plan_info {
   int mfma_one;         /* the same on the one-event shape (both count every tile on the MFMA form) */
   double samples_kept;      /* (cell, rapidity, azimuth) samples the reference's masks keep */
   double samples_evaluated; /* samples the stream kernel evaluates (3-D: whole tiles per reached sample) */
+  float est_batch_us;       /* estimated device time of the event inside a batch on the context's path
+                               (the serving loop's launch cap uses it) */
 } fsm_plan_info;
 int fsm_plan_describe(fsm_ctx *ctx, const fsm_params *p, fsm_plan_info *info);
 
@@ -247,6 +249,36 @@ int fsm_stream_tmunu_device(fsm_ctx *ctx, const fsm_params *p,
 int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T,
                       float *d_e, float *d_u, float *d_p, float *d_Pi,
                       float *d_pi, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* The reference's stage functions one by one, on its host layout: every
+ * argument is an array of planes of DIM floats in host memory, one pointer
+ * per component (the float** of src/Memoryf.cpp:5-13), fm^-4.  Each call
+ * copies its inputs to the device, runs the stage on the context's path and
+ * copies its outputs back before returning (synchronous).  p must have DIM
+ * and TAU set (fsm_params_finalize).  freestream-milne_amd/csrc/fsm_stages.h
+ * wraps them under the reference's own names and signatures. */
+
+/* solveEigenSystem(float **stressTensor, float *energyDensity,
+ * float **flowVelocity, parameters), src/LandauMatch.cpp:145-331: T (10
+ * planes, component order tt, tx, ty, tn, xx, xy, xn, yy, yn, nn) -> e, u (4
+ * planes), with the reference's selection rules and fallback. */
+int fsm_solve_eigen_system(fsm_ctx *ctx, const fsm_params *p, const float *const T[10], float *e,
+                           float *const u[4]);
+/* calculatePressure(float *energyDensity, float *baryonDensity,
+ * float *pressure, parameters), src/EquationOfState.cpp:5-80 (EOS_TYPE 1, 2,
+ * 3); nB is not read by any of them and may be NULL. */
+int fsm_calculate_pressure(fsm_ctx *ctx, const fsm_params *p, const float *e, const float *nB, float *pr);
+/* calculateBulkPressure(float **stressTensor, float *energyDensity,
+ * float *pressure, float *bulkPressure, parameters),
+ * src/LandauMatch.cpp:332-344. */
+int fsm_calculate_bulk_pressure(fsm_ctx *ctx, const fsm_params *p, const float *const T[10], const float *e,
+                                const float *pr, float *Pi);
+/* calculateShearViscTensor(float **stressTensor, float *energyDensity,
+ * float **flowVelocity, float *pressure, float *bulkPressure,
+ * float **shearTensor, parameters), src/LandauMatch.cpp:345-367. */
+int fsm_calculate_shear_visc_tensor(fsm_ctx *ctx, const fsm_params *p, const float *const T[10], const float *e,
+                                    const float *const u[4], const float *pr, const float *Pi, float *const pi[10]);
 
 /* ------------------------------------------------------------------ */
 /* Batches: many independent events (any mix of shapes, 2-D and 3-D) run with

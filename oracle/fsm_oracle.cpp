@@ -749,6 +749,34 @@ struct Pipeline {
   }
 };
 
+// calculateBulkPressure (src/LandauMatch.cpp:332-344), then
+// calculateShearViscTensor (:345-367), on planes of DIM values.
+template <typename R>
+void viscous_stages(int DIM, R TAU, const R *T, const R *e, const R *u, const R *p, R *Pi, R *pi) {
+  for (int is = 0; is < DIM; ++is) {
+    R a = T[0 * (size_t)DIM + is] - T[4 * (size_t)DIM + is] - T[7 * (size_t)DIM + is] -
+          TAU * TAU * T[9 * (size_t)DIM + is];
+    Pi[is] = (R)((-1.0 / 3.0) * (double)(a - e[is]) - (double)p[is]);
+  }
+  for (int is = 0; is < DIM; ++is) {
+    R b = e[is] + p[is] + Pi[is];
+    R c = p[is] + Pi[is];
+    const R *uc[4] = {&u[0 * (size_t)DIM + is], &u[1 * (size_t)DIM + is], &u[2 * (size_t)DIM + is],
+                      &u[3 * (size_t)DIM + is]};
+    auto Tc = [&](int k) { return T[(size_t)k * DIM + is]; };
+    pi[0 * (size_t)DIM + is] = Tc(0) - *uc[0] * *uc[0] * b + c;
+    pi[1 * (size_t)DIM + is] = Tc(1) - *uc[0] * *uc[1] * b;
+    pi[2 * (size_t)DIM + is] = Tc(2) - *uc[0] * *uc[2] * b;
+    pi[3 * (size_t)DIM + is] = Tc(3) - *uc[0] * *uc[3] * b;
+    pi[4 * (size_t)DIM + is] = Tc(4) - *uc[1] * *uc[1] * b - c;
+    pi[5 * (size_t)DIM + is] = Tc(5) - *uc[1] * *uc[2] * b;
+    pi[6 * (size_t)DIM + is] = Tc(6) - *uc[1] * *uc[3] * b;
+    pi[7 * (size_t)DIM + is] = Tc(7) - *uc[2] * *uc[2] * b - c;
+    pi[8 * (size_t)DIM + is] = Tc(8) - *uc[2] * *uc[3] * b;
+    pi[9 * (size_t)DIM + is] = (R)((double)(Tc(9) - *uc[3] * *uc[3] * b) - (double)c * (1.0 / (double)(TAU * TAU)));
+  }
+}
+
 template <typename R>
 int run_pipeline(const fsm_params &P, const float *e_in, const float *nB_in, double *out17,
                  double *T_out, double *extra, double *stage_seconds) {
@@ -793,31 +821,7 @@ int run_pipeline(const fsm_params &P, const float *e_in, const float *nB_in, dou
     for (int c = 0; c < 4; ++c) u[(size_t)c * DIM + is] = uu[c];
   }
   for (int is = 0; is < DIM; ++is) p[is] = pressure_of<R>(e[is], P.EOS_TYPE);
-  if (P.VISCOUS_MATCHING) {
-    for (int is = 0; is < DIM; ++is) {  // src/LandauMatch.cpp:332-344
-      R a = T[0 * (size_t)DIM + is] - T[4 * (size_t)DIM + is] - T[7 * (size_t)DIM + is] -
-            TAU * TAU * T[9 * (size_t)DIM + is];
-      Pi[is] = (R)((-1.0 / 3.0) * (double)(a - e[is]) - (double)p[is]);
-    }
-    for (int is = 0; is < DIM; ++is) {  // src/LandauMatch.cpp:345-367
-      R b = e[is] + p[is] + Pi[is];
-      R c = p[is] + Pi[is];
-      const R *uc[4] = {&u[0 * (size_t)DIM + is], &u[1 * (size_t)DIM + is], &u[2 * (size_t)DIM + is],
-                        &u[3 * (size_t)DIM + is]};
-      auto Tc = [&](int k) { return T[(size_t)k * DIM + is]; };
-      pi[0 * (size_t)DIM + is] = Tc(0) - *uc[0] * *uc[0] * b + c;
-      pi[1 * (size_t)DIM + is] = Tc(1) - *uc[0] * *uc[1] * b;
-      pi[2 * (size_t)DIM + is] = Tc(2) - *uc[0] * *uc[2] * b;
-      pi[3 * (size_t)DIM + is] = Tc(3) - *uc[0] * *uc[3] * b;
-      pi[4 * (size_t)DIM + is] = Tc(4) - *uc[1] * *uc[1] * b - c;
-      pi[5 * (size_t)DIM + is] = Tc(5) - *uc[1] * *uc[2] * b;
-      pi[6 * (size_t)DIM + is] = Tc(6) - *uc[1] * *uc[3] * b;
-      pi[7 * (size_t)DIM + is] = Tc(7) - *uc[2] * *uc[2] * b - c;
-      pi[8 * (size_t)DIM + is] = Tc(8) - *uc[2] * *uc[3] * b;
-      pi[9 * (size_t)DIM + is] = (R)((double)(Tc(9) - *uc[3] * *uc[3] * b) -
-                                     (double)c * (1.0 / (double)(TAU * TAU)));
-    }
-  }
+  if (P.VISCOUS_MATCHING) viscous_stages<R>(DIM, TAU, T.data(), e.data(), u.data(), p.data(), Pi.data(), pi.data());
   double t3 = now();
   // Output pack src/FreestreamMilne.cpp:695-718.
   if (out17) {
@@ -933,6 +937,13 @@ int oracle_params_finalize(fsm_params *p, const float *e_in, long n) {
 void oracle_pressure(int eos, int precision, long n, const double *e, double *p) {
   for (long i = 0; i < n; ++i)
     p[i] = precision == 0 ? (double)pressure_of<float>((float)e[i], eos) : pressure_of<double>(e[i], eos);
+}
+
+// The reference float path's calculateBulkPressure and
+// calculateShearViscTensor on given float planes (T 10, e, u 4, p): Pi, pi 10.
+void oracle_viscous_float(const fsm_params *P, const float *T, const float *e, const float *u, const float *p,
+                          float *Pi, float *pi) {
+  viscous_stages<float>(P->DIM, P->TAU, T, e, u, p, Pi, pi);
 }
 
 // Plants mistake `m` (enum Mutation) for the tests of the pins; 0 restores

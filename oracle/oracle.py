@@ -163,6 +163,23 @@ def pressure(e, eos: int, precision: str = "float"):
     return out
 
 
+def viscous_float(p, T, e, u, pr):
+    """The reference float path's calculateBulkPressure and
+    calculateShearViscTensor on given float planes: (Pi, pi[10])."""
+    lib = _lib()
+    from fsm_amd import Params
+
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.oracle_viscous_float.argtypes = [ctypes.POINTER(Params), fp, fp, fp, fp, fp, fp]
+    lib.oracle_viscous_float.restype = None
+    D = p.DIM
+    a = [np.ascontiguousarray(x, dtype=np.float32).ravel() for x in (T, e, u, pr)]
+    Pi = np.zeros(D, dtype=np.float32)
+    pi = np.zeros(10 * D, dtype=np.float32)
+    lib.oracle_viscous_float(ctypes.byref(p), *[_f(x) for x in a], _f(Pi), _f(pi))
+    return Pi, pi.reshape(10, D)
+
+
 class mutation:
     """Context manager planting one mistake in the oracle (see the Mutation
     ids in fsm_oracle.cpp), used only by the tests that show a pin fails on

@@ -51,6 +51,22 @@ def tmunu_err(T_gpu, T_ref, keep=None):
     return d.max() / scale
 
 
+def tmunu_local_err(T_gpu, T_ref, out_ref):
+    """Largest |dT^{mu nu}| relative to the cell's own T^{tau tau}, over the
+    cells with e > 1e-3 e_max (out_ref: the reference's 17 outputs).  The
+    global scale of tmunu_err hides an error in a dilute cell; this one does
+    not."""
+    T_gpu = np.asarray(T_gpu, dtype=np.float64).reshape(10, -1)
+    T_ref = np.asarray(T_ref, dtype=np.float64).reshape(10, -1)
+    e = np.asarray(out_ref, dtype=np.float64).reshape(17, -1)[0]
+    m = e > 1e-3 * e.max()
+    d = np.abs(T_gpu - T_ref)[:, m].max(axis=0)
+    t00 = np.abs(T_ref[0, m])
+    # a cell that keeps no sample (T = 0, the reference's fallback frame) must be exactly 0
+    rel = np.where(t00 > 0, d / np.where(t00 > 0, t00, 1.0), np.where(d > 0, np.inf, 0.0))
+    return float(rel.max()) if rel.size else 0.0
+
+
 def interior_mask(q, margin: int = 2):
     """Cells at least `margin` cells from every grid edge (x, y and, in 3-D,
     eta).  Next to an edge a sample whose exact position is a grid line sits on
@@ -115,10 +131,12 @@ def output_errs(out_gpu, out_ref, T00max, cond=None):
 
 # Largest fraction of the cells with e > 1e-3 e_max that the conditioning
 # filter may leave out of the strict e/u comparison, per path (2-D; 3-D grids
-# of 21 or fewer eta planes; other 3-D grids).  Dilute 2-D events with a
-# long streaming ring (DTAU/DX ~ 15) reach ~5% at kappa 100 and ~12% at 50.
-EXCLUDED_BUDGET = {"fp32": {"2d": 0.06, "3d_thin": 0.25, "3d": 0.12},
-                   "mixed": {"2d": 0.15, "3d_thin": 0.35, "3d": 0.20}}
+# of 21 or fewer eta planes; other 3-D grids).  The fraction grows with the
+# streaming ring's radius DTAU/DX: over the 512-event 2-D mix the largest is
+# 11% at kappa 100 and 19% at 50 (301^2 events with DTAU/DX ~ 17), C1 (16.7)
+# has 1.1% and 3.8%; C2-style 3-D events 4% and 9%.
+EXCLUDED_BUDGET = {"fp32": {"2d": 0.15, "3d_thin": 0.25, "3d": 0.12},
+                   "mixed": {"2d": 0.25, "3d_thin": 0.35, "3d": 0.20}}
 
 
 def excluded_budget(q, path="fp32"):

@@ -186,3 +186,41 @@ def test_calls_on_two_streams_are_ordered(gpu_ctx):
             gpu_ctx.run_device(qb, db.data_ptr(), ob.data_ptr(), s2)
         torch.cuda.synchronize()
         assert torch.equal(oa.cpu(), refs[0]) and torch.equal(ob.cpu(), refs[1]), rep
+
+
+@pytest.mark.parametrize("path", ["fp32", "mixed"])
+def test_cost_estimate_tracks_measured_batch_time(path):
+    """The serving loop caps each launch by the plan's estimate of an event's
+    device time inside a batch (fsm_plan_info.est_batch_us); it must be the
+    measured time to within a factor of three, for 2-D and 3-D events."""
+    import time
+
+    import torch
+
+    ctx = fsm_amd.Context(0)
+    ctx.set_path(fsm_amd.PATH_MIXED if path == "mixed" else fsm_amd.PATH_FP32)
+    try:
+        for name, B in (("C1", 64), ("C2", 2)):
+            p, e = W.config(name)
+            q = fsm_amd.params_finalize(p, e)
+            d_e = torch.from_numpy(e).cuda()
+            outs = [torch.empty(17 * q.DIM, dtype=torch.float64, device="cuda") for _ in range(B)]
+            b = fsm_amd.Batch(ctx)
+            for o in outs:
+                b.add(q, d_e.data_ptr(), o.data_ptr())
+            s = torch.cuda.current_stream()
+            for _ in range(3):
+                b.run(s)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 10
+            for _ in range(reps):
+                b.run(s)
+            torch.cuda.synchronize()
+            measured = (time.perf_counter() - t0) / (reps * B) * 1e6
+            est = ctx.plan_info(q)["est_batch_us"]
+            print(name, path, "estimated %.1f us per event, measured %.1f us" % (est, measured), flush=True)
+            assert measured / 3 <= est <= 3 * measured, (name, est, measured)
+            b.close()
+    finally:
+        ctx.close()

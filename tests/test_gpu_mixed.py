@@ -13,7 +13,8 @@ import pytest
 import fsm_amd
 import fsm_amd.workloads as W
 import oracle as O
-from parity import TOL, excluded_cells_check, interior_mask, output_errs, ring_interior_mask, tmunu_err, well_conditioned
+from parity import (TOL, excluded_cells_check, interior_mask, output_errs, ring_interior_mask, tmunu_err, tmunu_local_err,
+                    well_conditioned)
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +63,10 @@ def _case(name):
     if name == "int2d":  # DTAU/DX = 10: samples on grid lines next to the mask edge
         p = W.base_params(DIM_X=101, DIM_Y=101, DIM_PHIP=64, DX=0.08, DY=0.08, TAU0=0.1, DTAU=0.8)
         return p, W.hotspots_2d(101, 101, 0.08, 0.08, 60, 13).ravel()
+    if name == "lastline2d":  # kept samples on the last knot line take the reference's bilinear
+        from test_gpu_parity import last_knot_line_event
+
+        return last_knot_line_event()
     return W.config(name)
 
 
@@ -80,7 +85,7 @@ def _gpu(ctx, q, e):
 
 
 @pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "trento_ref", "small3d",
-                                  "small3d_int", "rect3d", "wide3d"])
+                                  "small3d_int", "rect3d", "wide3d", "lastline2d"])
 def test_mixed_parity(mixed_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)
@@ -89,6 +94,9 @@ def test_mixed_parity(mixed_ctx, name):
     ref = O.run(q, e, "float")
     dref = O.run(q, e, "double")
     tf = tmunu_err(T, ref["T"])
+    tl = tmunu_local_err(T, ref["T"], ref["out"])
+    print(name, "mixed T err per cell (relative to the cell's T^tt) %.3g" % tl, flush=True)
+    assert tl <= TOL, tl
     inner = ring_interior_mask(q) if q.DIM_ETA == 1 else interior_mask(q)
     td = tmunu_err(T, dref["T"], keep=inner)
     T00max = np.abs(ref["T"][0]).max()

@@ -65,26 +65,74 @@ def test_results_alone_and_in_the_mix(gpu_ctx):
     srv.close()
 
 
+def _hold_stream(s):
+    """Queue tens of milliseconds of ordinary work (matrix products) on s:
+    device-input submits on s upload only after it, so the server's launches
+    wait on the device while the next requests arrive on the host."""
+    import torch
+
+    with torch.cuda.stream(s):
+        a = torch.full((4096, 4096), 1.0e-3, dtype=torch.float32, device="cuda")
+        for _ in range(24):
+            a = torch.mm(a, a).clamp_(-1.0, 1.0)
+    return a
+
+
 def test_arrivals_during_running_work_are_batched(gpu_ctx):
+    """Requests that arrive while a launch is in flight queue up and leave
+    together, up to the launch's device-time cap (one launch slot)."""
+    import torch
+
     p, e = W.config("C1")
-    srv = fsm_amd.Server(gpu_ctx, max_batch_events=64, max_inflight=2)
+    srv = fsm_amd.Server(gpu_ctx, max_batch_events=64, max_inflight=1)
+    s = torch.cuda.current_stream()
+    d_e = torch.from_numpy(e).cuda()
     # first round fills the memory pools (the first allocations are slow)
-    warm = [srv.submit(p, e) for _ in range(40)]
+    warm = [srv.submit_device(p, d_e.data_ptr(), e.size, stream=s) for _ in range(40)]
     for _ in warm:
-        srv.release(srv.poll(2000000))
+        srv.release(srv.poll(5000000))
     launches0 = srv.stats()["launches"]
-    ids = [srv.submit(p, e) for _ in range(40)]
+    keep = _hold_stream(s)
+    ids = [srv.submit_device(p, d_e.data_ptr(), e.size, stream=s) for _ in range(40)]
     got = 0
     while got < len(ids):
-        if srv.poll(500000) is not None:
+        if srv.poll(2000000) is not None:
             got += 1
     st = srv.stats()
-    # 40 requests arriving faster than one event runs are launched in fewer
-    # launches than requests
-    assert st["launches"] - launches0 < len(ids)
+    # the first request launches alone; the other 39 arrive while it waits
+    assert st["launches"] - launches0 < len(ids) // 2, st
     ref = _alone(gpu_ctx, [(p, e)])[0]
     for rid in ids:
         _same(srv.fetch(rid), ref, np.abs(ref["e"]).max())
+    del keep
+    srv.close()
+
+
+def test_2d_request_does_not_wait_behind_3d_launches(gpu_ctx):
+    """Slot 0 takes only 2-D requests: with two launch slots, a 2-D request
+    submitted after three 3-D requests completes before the last of them."""
+    import torch
+
+    p, e = W.config("C1")
+    p3, e3 = W.c2_like(101, seed=5)
+    srv = fsm_amd.Server(gpu_ctx, max_inflight=2)
+    s = torch.cuda.current_stream()
+    d2, d3 = torch.from_numpy(e).cuda(), torch.from_numpy(e3).cuda()
+    warm = [srv.submit_device(p3, d3.data_ptr(), e3.size, stream=s), srv.submit_device(p, d2.data_ptr(), e.size, stream=s)]
+    for _ in warm:
+        srv.release(srv.poll(5000000))
+    keep = _hold_stream(s)
+    three = [srv.submit_device(p3, d3.data_ptr(), e3.size, stream=s) for _ in range(3)]
+    two = srv.submit_device(p, d2.data_ptr(), e.size, stream=s)
+    order = []
+    while len(order) < 4:
+        rid = srv.poll(2000000)
+        assert rid is not None
+        order.append(rid)
+    assert order.index(two) < order.index(three[-1]), order
+    for rid in order:
+        srv.release(rid)
+    del keep
     srv.close()
 
 
