@@ -191,6 +191,15 @@ void set_variant(Event2D &ev, int v) {
 
 }  // namespace
 
+// Workspace of fsm_run_device_noalloc: a fixed device array instead of
+// allocations (one copy per GPU, alive until the process ends).  Host code
+// uses only the address hipGetSymbolAddress returns for it, kept per device.
+#define FSM_NOALLOC_SCRATCH_BYTES ((size_t)80 << 20)
+#define FSM_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[FSM_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[FSM_NOALLOC_MAX_DEVICES];
+
 struct fsm_ctx {
   int device = 0;
   int path = FSM_PATH_FP32;
@@ -219,6 +228,8 @@ struct fsm_ctx {
   std::recursive_mutex call_mu;  // fsm_run holds it around its nested fsm_run_device
   hipEvent_t last_call = nullptr;
   bool last_call_recorded = false;
+  // device address of `scratch` on this context's device (null if the lookup failed)
+  void *noalloc_scratch = nullptr;
 };
 
 namespace {
@@ -1149,6 +1160,8 @@ const char *fsm_status_string(int status) {
     case FSM_ERR_UNSUPPORTED: return "parameter combination not supported";
     case FSM_ERR_BUSY: return "no free slot";
     case FSM_ERR_CANCELLED: return "cancelled";
+    case FSM_ERR_TOO_LARGE: return "event exceeds the fixed workspace of fsm_run_device_noalloc";
+    case FSM_ERR_NO_PLAN: return "event shape not prepared: no tables for it in this context yet";
     default: return "unknown status";
   }
 }
@@ -1169,6 +1182,17 @@ int fsm_create(int device, fsm_ctx **out) {
   ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   FSM_TRY(fsm_set_lds_limits(lds, lds));
   FSM_TRY(hipEventCreateWithFlags(&ctx->last_call, hipEventDisableTiming));
+  if (device < FSM_NOALLOC_MAX_DEVICES) {
+    // once per device, with that device current: the address of its copy of
+    // the allocation-free entry point's workspace
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void *ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = ptr;
+      else (void)hipGetLastError();  // fsm_run_device_noalloc then reports FSM_ERR_DEVICE
+    }
+    ctx->noalloc_scratch = g_scratch_ptr[device];
+  }
   *out = ctx.release();
   return FSM_OK;
 }
@@ -2268,6 +2292,62 @@ int batch_run_graph_or_direct(fsm_batch *b, hipStream_t s) {
   return FSM_OK;
 }
 }  // namespace
+
+
+namespace {
+// The cached plan of p's shape, or null: no table is built and the cache is
+// not reordered (both could allocate).
+std::shared_ptr<Plan> find_plan(fsm_ctx *ctx, const fsm_params &p) {
+  std::lock_guard<std::mutex> lock(ctx->plan_mu);
+  for (const auto &pl : ctx->plans)
+    if (same_shape(pl->key, p)) return pl;
+  return nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, double *d_out, void *stream) {
+  if (!ctx || !p || !d_e_GeV || !d_out) return FSM_ERR_ARG;
+  FSM_TRY(hipSetDevice(ctx->device));
+  if (!ctx->noalloc_scratch) return FSM_ERR_DEVICE;
+  if (event_workspace_bytes(*p) > FSM_NOALLOC_SCRATCH_BYTES) return FSM_ERR_TOO_LARGE;
+  hipStream_t s = (hipStream_t)stream;
+  CallOrder order(ctx, s);
+  FSM_TRY(order.begin());
+  std::shared_ptr<Plan> plan = find_plan(ctx, *p);
+  if (!plan) return FSM_ERR_NO_PLAN;
+  const Plan &pl = *plan;
+  char *ws = (char *)ctx->noalloc_scratch;
+  const int mixed = ctx->path == FSM_PATH_MIXED;
+  EventLandau L;
+  if (!pl.is3d) {
+    // the one-event kernels of fsm_run_device, with the spline planes, the
+    // fp16 planes, the tile flags, the knot scale and T in the workspace
+    const int v = ctx->tile_policy == FSM_TILES_BATCH ? pl.bvariant[ctx->path] : pl.ev2.variant;
+    char *cur = ws;
+    Event2D ev = event2d_in_workspace(pl, *p, d_e_GeV, ctx->path, v, cur);
+    FSM_TRY(fsm_launch_spline2d(ev, s));
+    FSM_TRY(fsm_launch_stream2d(ev, s));
+    L = landau_event(*p, ev.T, 0, mixed);
+    L.eta_zero = 1;
+  } else {
+    // source planes and eta-major T in the workspace
+    Event3D ev = pl.ev3;
+    ev.e_gev = d_e_GeV;
+    ev.src = (float *)ws;
+    ev.T = (float *)(ws + align256((size_t)p->DIM * sizeof(float)));
+    ev.t_plane = p->DIM;
+    FSM_TRY(fsm_launch_src3d(ev, s));
+    FSM_TRY(fsm_launch_stream3d(ev, s));
+    L = landau_event(*p, ev.T, 1, mixed);
+  }
+  L.out = d_out;
+  FSM_TRY(fsm_launch_landau(L, s));
+  return FSM_OK;
+}
+
+}  // extern "C"
 
 // ---- internal entry points for the server (fsm_host.h) ----------------------
 namespace fsm_host {

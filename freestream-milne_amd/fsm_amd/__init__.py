@@ -44,6 +44,8 @@ FSM_ERR_ARG = -6
 FSM_ERR_UNSUPPORTED = -7
 FSM_ERR_BUSY = -8
 FSM_ERR_CANCELLED = -9
+FSM_ERR_TOO_LARGE = -10
+FSM_ERR_NO_PLAN = -11
 
 PATH_FP32 = 0
 PATH_MIXED = 1
@@ -175,6 +177,7 @@ def _lib():
         lib.fsm_run_baryon.argtypes = [vp, P, fp, fp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), fp]
         lib.fsm_run_device.argtypes = [vp, P, vp, vp, vp]
+        lib.fsm_run_device_noalloc.argtypes = [vp, P, vp, vp, vp]
         lib.fsm_run_baryon_device.argtypes = [vp, P, vp, vp, vp, vp]
         lib.fsm_stream_tmunu_device.argtypes = [vp, P, vp, vp, vp]
         lib.fsm_landau_device.argtypes = [vp, P, vp, vp, vp, vp, vp, vp, vp]
@@ -384,6 +387,11 @@ class Context:
     def run_device(self, p: Params, d_e: int, d_out: int, stream=None):
         _check(_lib().fsm_run_device(self._ctx, ctypes.byref(p), d_e, d_out, _stream_handle(stream)),
                "fsm_run_device")
+
+    def run_device_noalloc(self, p: Params, d_e: int, d_out: int, stream=None) -> int:
+        """fsm_run_device_noalloc: returns the status (FSM_ERR_TOO_LARGE and
+        FSM_ERR_NO_PLAN are not raised, so callers can test for them)."""
+        return _lib().fsm_run_device_noalloc(self._ctx, ctypes.byref(p), d_e, d_out, _stream_handle(stream))
 
     def run_baryon_device(self, p: Params, d_e: int, d_nB: int, d_out: int, stream=None):
         """d_out: NUM_OUTPUTS_BARYON planes of DIM doubles (17 outputs, n_B, V^mu)."""

@@ -78,7 +78,9 @@ enum {
   FSM_ERR_ARG = -6,         /* invalid argument (NULL pointer, bad size) */
   FSM_ERR_UNSUPPORTED = -7, /* parameter combination outside this library */
   FSM_ERR_BUSY = -8,        /* serving queue full / no slot free */
-  FSM_ERR_CANCELLED = -9    /* request cancelled before it ran */
+  FSM_ERR_CANCELLED = -9,   /* request cancelled before it ran */
+  FSM_ERR_TOO_LARGE = -10,  /* event larger than the fixed workspace of fsm_run_device_noalloc */
+  FSM_ERR_NO_PLAN = -11     /* fsm_run_device_noalloc: the event shape has no tables in the context yet */
 };
 
 /* Number and order of the OUTPUTFORMAT=2 result vectors:
@@ -232,6 +234,21 @@ int fsm_run_baryon_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_Ge
  * of DIM in the output order).  Equivalent to fsm_run without host copies. */
 int fsm_run_device(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV,
                    double *d_out, void *stream);
+
+/* fsm_run_device without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as fsm_run_device; its event-sized workspace is a fixed
+ * 80 MiB device array that every GPU the library runs on holds until the
+ * process ends (2-D events up to about 1.1 M cells, 3-D events up to about
+ * 1.9 M cells: the 131^2 x 101 event needs 72.7 MiB); a larger event returns
+ * FSM_ERR_TOO_LARGE.  The event shape's tables must exist in the context
+ * already (any earlier call with that shape builds them, e.g.
+ * fsm_plan_describe or fsm_run_device), else FSM_ERR_NO_PLAN.  Every call on
+ * one GPU shares that array: calls on one context are ordered as the other
+ * one-event calls are, but calls through different contexts on the same GPU
+ * must not overlap (they would race on the array). */
+int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_GeV, double *d_out, void *stream);
 
 /* Stage A: convertInitialDensity + freeStream + calculateHypertrigTable +
  * calculateStressTensor fused (src/FreeStream.cpp:97-350,
