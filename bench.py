@@ -395,7 +395,7 @@ def main():
         "note": "achieved = the reference's 52 FLOP per (cell, phi) sample x %d samples per launch (%d events) / "
                 "the launch's average time (HIP events on the launch stream), against MI355X's dense FP16 MFMA "
                 "peak; executed_mfma counts the v_mfma_f32_16x16x32_f16 instructions the kernel issues (%d per "
-                "event, 16384 FLOP each: hi+lo triple product, 12 of 16 columns used), %.1fx the algorithmic "
+                "event, 16384 FLOP each: knots and weights both split hi+lo, all four products, 12 of 16 columns used), %.1fx the algorithmic "
                 "work; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"
                 % (samples, B, info["mfma_batch"], mfma_flops / alg_flops),
     }

@@ -624,6 +624,7 @@ int build_plan2d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.st = pl.st.p;
   ev.groups = pl.groups.p;
   ev.t_plane = p.DIM;
+  ev.t_row = ny;
   ev.mfma = 0;  // set per call from the context's path
   ev.n_moff = (int)mt.offs.size();
   ev.n_mgroups = (int)mt.groups.size();
@@ -1835,9 +1836,13 @@ size_t event_workspace_bytes_all(const fsm_params &p, bool baryon) {
   return event_workspace_bytes(p) * (baryon ? 2 : 1);
 }
 
+// Row pitch of a batch's internal 2-D T (cells): DIM_Y rounded up to 32.
+inline int padded_row(int ny) { return (ny + 31) & ~31; }
+
 size_t event_workspace_bytes(const fsm_params &p) {
   if (p.DIM_ETA == 1)
-    return align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4)) + align256((size_t)10 * p.DIM * sizeof(float)) +
+    return align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4)) +
+           align256((size_t)10 * p.DIM_X * padded_row(p.DIM_Y) * sizeof(float)) +
            align256((size_t)p.DIM_X * p.DIM_Y * sizeof(uint4)) + align256(spline_flag_count(p) * sizeof(unsigned)) +
            align256((FSM_KS_PARTS + 1) * sizeof(float));
   return align256((size_t)p.DIM * sizeof(float)) + align256((size_t)10 * p.DIM * sizeof(float));
@@ -1845,17 +1850,20 @@ size_t event_workspace_bytes(const fsm_params &p) {
 
 // Descriptor of one 2-D event of a batch whose workspace starts at `cur`
 // (layout of event_workspace_bytes): spline planes, T, MFMA planes, spline
-// tile flags, knot scale.  Advances `cur` past the workspace.
+// tile flags, knot scale.  Advances `cur` past the workspace.  pad_rows: T
+// rows at the padded pitch (read through EventLandau::t_row), else the
+// reference layout.
 Event2D event2d_in_workspace(const Plan &pl, const fsm_params &p, const float *d_e, int path, int variant,
-                             char *&cur) {
+                             char *&cur, bool pad_rows) {
   Event2D d = pl.ev2;
   d.e_gev = d_e;
   set_variant(d, variant);
   d.planes = (float4 *)cur;
   cur += align256((size_t)p.DIM_X * p.DIM_Y * sizeof(float4));
   d.T = (float *)cur;
-  cur += align256((size_t)10 * p.DIM * sizeof(float));
-  d.t_plane = p.DIM;
+  cur += align256((size_t)10 * p.DIM_X * padded_row(p.DIM_Y) * sizeof(float));
+  d.t_row = pad_rows ? padded_row(p.DIM_Y) : p.DIM_Y;
+  d.t_plane = (long)p.DIM_X * d.t_row;
   uint4 *ph = (uint4 *)cur;
   cur += align256((size_t)p.DIM_X * p.DIM_Y * sizeof(uint4));
   d.sp_flags = (unsigned *)cur;
@@ -1904,7 +1912,7 @@ int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p) {
     char *cur = ws.p;
     const int sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
     for (int i = 0; i < n; ++i) {
-      Event2D d = event2d_in_workspace(pl, p, d_in.p, path, v, cur);
+      Event2D d = event2d_in_workspace(pl, p, d_in.p, path, v, cur, true);
       if (d.mfma)
         for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
       for (int x = 0; x < (nx + FSMSP2_TX - 1) / FSMSP2_TX; ++x)
@@ -2049,7 +2057,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
   auto add_pass = [&](const BatchEvent &ev, const float *d_in, bool charge, char *&cur, int *eta_major) {
     const fsm_params &p = ev.p;
     if (!ev.plan->is3d) {
-      Event2D d = event2d_in_workspace(*ev.plan, p, d_in, path, ev.plan->bvariant[path], cur);
+      Event2D d = event2d_in_workspace(*ev.plan, p, d_in, path, ev.plan->bvariant[path], cur, true);
       // the batch's T is read only by its 3x3 Landau launch (k_landau2d_batch)
       d.skip_eta_planes = charge ? 0 : 1;
       if (charge) {
@@ -2103,6 +2111,12 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     const float *T = add_pass(ev, ev.d_e, false, cur, &em);
     EventLandau L = landau_event(p, T, em, path == FSM_PATH_MIXED);
     L.eta_zero = p.DIM_ETA == 1;  // the 2-D stream kernel writes the eta components as zeros
+    if (!em) {
+      // the batch's 2-D T (and J) rows sit at the padded pitch
+      L.t_ny = p.DIM_Y;
+      L.t_row = padded_row(p.DIM_Y);
+      L.t_plane = (long)p.DIM_X * L.t_row;
+    }
     L.out = ev.d_out;
     if (ev.d_nb) {
       L.J = add_pass(ev, ev.d_nb, true, cur, &jm);
@@ -2326,7 +2340,7 @@ int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_G
     // fp16 planes, the tile flags, the knot scale and T in the workspace
     const int v = ctx->tile_policy == FSM_TILES_BATCH ? pl.bvariant[ctx->path] : pl.ev2.variant;
     char *cur = ws;
-    Event2D ev = event2d_in_workspace(pl, *p, d_e_GeV, ctx->path, v, cur);
+    Event2D ev = event2d_in_workspace(pl, *p, d_e_GeV, ctx->path, v, cur, false);
     FSM_TRY(fsm_launch_spline2d(ev, s));
     FSM_TRY(fsm_launch_stream2d(ev, s));
     L = landau_event(*p, ev.T, 0, mixed);

@@ -169,8 +169,10 @@ struct Event2D {
   int cylo, cyhi;       // columns [cylo, cyhi] of gy likewise
   const Stencil2D *st;  // [DIM_PHIP]
   const StencilGroup *groups;
-  float *T;             // 10 planes, reference layout
-  long t_plane;         // plane stride of T (>= DIM)
+  float *T;             // 10 planes: cell (ix, iy) at ix * t_row + iy
+  long t_plane;         // plane stride of T (>= nx * t_row)
+  int t_row;            // row pitch of T in cells: ny (the reference layout), or ny rounded up to 32
+                        // (a batch's internal T: every tile's 32-row run starts on a 128-byte line)
   // ---- mixed-precision quadrature (MFMA tiles) ----
   int mfma;             // 1: MFMA tile kernel; spline kernel also writes planes_h and sp_flags
   int n_moff;           // knot offsets carrying at least one MFMA sample
@@ -305,6 +307,8 @@ struct EventLandau {
   double inv_tau2;      // 1.0 / (double)(TAU * TAU), the per-event factor of pi^{eta eta} (src/LandauMatch.cpp:365)
   const float *T;
   long t_plane;
+  int t_ny, t_row;      // 2-D T with a padded row pitch: cell is = ix * t_ny + iy at ix * t_row + iy
+                        // (t_row = 0: T is in the reference layout or eta-major)
   double *out;          // 17 planes of dim doubles (may be null)
   long out_plane;
   float *fe, *fu, *fp, *fPi, *fpi;  // optional float outputs

@@ -1184,6 +1184,9 @@ __device__ void landau_cell_out(const EventLandau &ev, bool live, long is, Landa
     const long xy = is / ev.neta;
     const long ie = is - xy * ev.neta;
     tidx = ie * ev.nxy + xy;
+  } else if (ev.t_row) {
+    const long ix = is / ev.t_ny;
+    tidx = ix * ev.t_row + (is - ix * ev.t_ny);
   }
   CellOut r;
   landau_cell_compute<N>(ev, live, tidx, r, S);

@@ -237,12 +237,12 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
     const int ix = ixb + m;
     if (ix < nx && iy < ny && ev.out_j) {
       // src/LandauMatch.cpp:140: J = sum * DRAP * d_phip; rows 1, c, s, tanh(0) = 0
-      const long is = (long)ix * ny + iy;
+      const long is = (long)ix * ev.t_row + iy;
 #pragma unroll
       for (int c = 0; c < 3; ++c) ev.T[c * P + is] = ((acc[m][c] * ev.out_scale) * ev.jscale) * ev.d_phip;
       ev.T[3 * P + is] = 0.0f;
     } else if (ix < nx && iy < ny) {
-      const long is = (long)ix * ny + iy;
+      const long is = (long)ix * ev.t_row + iy;
       float r[6];
       const float s5 = acc[m][0] - acc[m][3];
 #pragma unroll
@@ -666,7 +666,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     if (!cell_live(q)) continue;
     if (ev.out_j) {
       // src/LandauMatch.cpp:140: J = sum * DRAP * d_phip; rows 1, c, s, tanh(0) = 0
-      const long is = (long)cell_x(q) * ny + cell_y(q);
+      const long is = (long)cell_x(q) * ev.t_row + cell_y(q);
 #pragma unroll
       for (int c = 0; c < 3; ++c) fsm_gstore(ev.T + c * P + is, ((t[q][c] * ev.out_scale) * ev.jscale) * ev.d_phip);
       fsm_gstore(ev.T + 3 * P + is, 0.0f);
@@ -675,7 +675,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     float rr[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) rr[m] = ((t[q][m] * ev.out_scale) * ev.d_phip) / ev.tau;
-    const long is = (long)cell_x(q) * ny + cell_y(q);
+    const long is = (long)cell_x(q) * ev.t_row + cell_y(q);
     fsm_gstore(ev.T + 0 * P + is, rr[0]);
     fsm_gstore(ev.T + 1 * P + is, rr[1]);
     fsm_gstore(ev.T + 2 * P + is, rr[2]);
