@@ -1217,28 +1217,31 @@ int fsm_set_path(fsm_ctx *ctx, int path) {
 namespace {
 // MFMA instructions of one event's MFMA stream stage on tile shape `variant`,
 // following the kernel's decisions (fsm_stream2d.hip, stream2d_mfma_tile): a
-// wave whose cells keep every sample of a group runs the group's pair MFMAs
-// for each of its XW columns and 2 row blocks; otherwise each offset's MFMAs
-// for the row blocks and columns whose floor indices can be in range.  Tiles
-// that fall back to the FP32 loop for spline undershoot are not subtracted.
+// wave with no live column runs none; a wave whose cells keep every sample of
+// a group runs the group's pair MFMAs for each of its XW columns and its row
+// blocks holding live cells; otherwise each offset's MFMAs for those row
+// blocks and the columns whose floor indices can be in range.  Tiles that
+// fall back to the FP32 loop for spline undershoot are not subtracted.
 long mfma_count(const Plan &pl, int variant, int nx, int ny) {
   const int TX = fsm_stream2d_tile_w(variant), TY = fsm_stream2d_tile_h(variant), XW = TX / 4;
   long n = 0;
   for (int x0 = 0; x0 < nx; x0 += TX)
     for (int y0 = 0; y0 < ny; y0 += TY) {
       const int ylast = std::min(y0 + TY - 1, ny - 1);
+      const int nyb = y0 + 16 < ny ? 2 : 1;
       for (int w = 0; w < 4; ++w) {
         const int xw0 = x0 + w * XW, xw1 = std::min(xw0 + XW - 1, nx - 1);
+        if (xw0 >= nx) continue;
         for (const StencilGroup &G : pl.mgroups_h) {
           const bool interior = (xw0 + G.bx0 >= 1) && (xw1 + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
                                 (ylast + G.by1 - 1 <= ny - 2);
           if (interior) {
-            n += (long)G.pad1 * XW * 2;
+            n += (long)G.pad1 * XW * nyb;
             continue;
           }
           for (int o = G.j0; o < G.j1; ++o) {
             const int ox = pl.moffs_h[o].x, oy = pl.moffs_h[o].y;
-            for (int yb = 0; yb < 2; ++yb) {
+            for (int yb = 0; yb < nyb; ++yb) {
               const int ylo = y0 + yb * 16 + oy;
               if (ylo + 15 < 1 || ylo > ny - 2) continue;
               for (int xc = 0; xc < XW; ++xc)

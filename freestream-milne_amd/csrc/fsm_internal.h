@@ -76,6 +76,14 @@ __device__ __forceinline__ uint2 fsm_gload(const uint2 *p) {
 }
 __device__ __forceinline__ unsigned fsm_gload(const unsigned *p) { return *(const FSM_AS_GLOBAL unsigned *)p; }
 __device__ __forceinline__ void fsm_gstore(float *p, float v) { *(FSM_AS_GLOBAL float *)p = v; }
+// Copies 16 bytes per lane from global memory straight into LDS
+// (global_load_lds_dwordx4): lane l of the wave writes lds_base[l], with
+// lds_base wave-uniform and 16-byte aligned.  Inactive lanes write nothing.
+// Counts in the vector-memory wait counter.
+__device__ __forceinline__ void fsm_lds_dma16(const void *src, void *lds_base) {
+  __builtin_amdgcn_global_load_lds((FSM_AS_GLOBAL void *)const_cast<void *>(src),
+                                   (__attribute__((address_space(3))) void *)lds_base, 16, 0, 0);
+}
 // Scalar load of a trivially copyable descriptor (size a multiple of 4 bytes).
 template <typename T>
 __device__ __forceinline__ T fsm_sload(const T *p) {

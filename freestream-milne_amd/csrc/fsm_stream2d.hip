@@ -346,13 +346,16 @@ __device__ __forceinline__ void bmat_expand(const uint4 *bl, int m, int lane, fs
 // MASKED: edge mask (src/FreeStream.cpp:213), exact here since the floor
 // index of an MFMA sample is ix + ox for every cell.  Rows out of range zero
 // their knot registers (a 16-row block is all in, all out, or split), columns
-// out of range skip their MFMAs; both tests are wave-uniform.
+// out of range skip their MFMAs; both tests are wave-uniform.  A second row
+// block with no live cell (y0 + 16 >= ny) is skipped in both forms.
 template <int XW, bool MASKED>
 __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
                                              const int4 *orec, int no, int lane_base, int LYP, int gx, int r,
                                              int xw0, int y0, int nx, int ny) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
+  const int nyb = y0 + 16 < ny ? 2 : 1;  // row blocks holding live cells
+  no = __builtin_amdgcn_readfirstlane(no);
   for (int o = 0; o < no; ++o) {
     const int4 rec = orec[o];
     const int ox = __builtin_amdgcn_readfirstlane(rec.x), oy = __builtin_amdgcn_readfirstlane(rec.y);
@@ -361,6 +364,7 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
     bmat_expand(bl, o, lane, b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
+      if (yb >= nyb) break;
       // one 16-row block at a time: the knot registers of a block are
       // reused by its XW columns and then released
       uint4 R[NR][1];
@@ -412,9 +416,11 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
 // group (no masks), so merging samples of different offsets is exact.
 template <int XW>
 __device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
-                                           const int4 *prec, int np, int lane_base, int LYP, int gx, int gy) {
+                                           const int4 *prec, int np, int lane_base, int LYP, int gx, int gy,
+                                           int nyb) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
+  np = __builtin_amdgcn_readfirstlane(np);
   for (int o = 0; o < np; ++o) {
     const int4 rec = prec[o];
     const int delta = gy ? rec.y : rec.x;
@@ -422,6 +428,7 @@ __device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint
     bmat_expand(bl, o, lane, b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
+      if (yb >= nyb) break;
       uint4 R[NR];
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
@@ -445,108 +452,45 @@ __device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint
   }
 }
 
-// Registers that carry one offset group's LDS image -- knot box, weights,
-// offset records -- from its global loads (issued before the previous
-// group's MFMAs, so they overlap them) to its LDS stores.  Elements past the
-// register capacity are copied directly at store time.
-// (register capacity of the prefetch; elements beyond it are copied at store
-// time -- kept small so three workgroups fit per CU)
-constexpr int kPfBox = 4, kPfW = 1;
-struct GroupPrefetch {
-  uint4 box[kPfBox];
-  uint4 w[kPfW];
-  int4 rec;
-};
+// A zero knot in global memory: the source of box slots off the grid and of
+// the padding rows of each box column.
+__device__ const uint4 fsm_zero_knot = {0u, 0u, 0u, 0u};
 
-// Walks box element k = threadIdx.x + u * blockDim.x as (column, row).
-struct BoxWalk {
-  int lx, ly, dlx, dly, LY;
-  __device__ __forceinline__ BoxWalk(int LY_) : LY(LY_) {
-    dlx = blockDim.x / LY;
-    dly = blockDim.x - dlx * LY;
-    lx = threadIdx.x / LY;
-    ly = threadIdx.x - lx * LY;
-  }
-  __device__ __forceinline__ void next() {
+// LDS image of one offset group, copied straight from global memory (16 B
+// per lane, a wave fills 64 consecutive slots): the knot box [LX][LYP]
+// (knots off the grid and rows LY..LYP-1 read as zero), then the packed B
+// operands of the group's MFMAs of both forms, then their records.  Wave w
+// fills chunks w, w + 4, ... of each part.
+__device__ __forceinline__ void group_dma(const Event2D &ev, const StencilGroup &G, uint4 *lh, int x0, int y0,
+                                          int TX, int TY, int wave, int lane) {
+  const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
+  const int n = LX * LYP;
+  const int no = G.j1 - G.j0 + G.pad1, nw = no * FSM_BMAT_U4;
+  const int bx = x0 + G.bx0, by = y0 + G.by0;
+  // slot s = 64 c + lane at box (lx, ly); s advances by 256 per chunk step
+  const int s0 = wave * 64 + lane;
+  int lx = s0 / LYP, ly = s0 - lx * LYP;
+  const int dlx = 256 / LYP, dly = 256 - dlx * LYP;
+  for (int c = wave; c * 64 < n; c += 4) {
+    const int kx = bx + lx, ky = by + ly;
+    const bool in = ly < LY && (unsigned)kx < (unsigned)ev.nx && (unsigned)ky < (unsigned)ev.ny;
+    const uint4 *src = in ? ev.planes_h + (kx * ev.ny + ky) : &fsm_zero_knot;
+    if (c * 64 + lane < n) fsm_lds_dma16(src, lh + c * 64);
     lx += dlx;
     ly += dly;
-    if (ly >= LY) {
-      ly -= LY;
+    if (ly >= LYP) {
+      ly -= LYP;
       ++lx;
     }
   }
-};
-
-__device__ __forceinline__ void group_load(GroupPrefetch &pf, const Event2D &ev, const StencilGroup *gp, int x0,
-                                           int y0, int TX, int TY) {
-  const int nx = ev.nx, ny = ev.ny;
-  // The descriptor's fields are needed before any load of the group is
-  // issued: the empty asm makes them available here, so that the wait for
-  // them cannot land after the box loads (and wait for those too).
-  int bx = x0 + gp->bx0, by = y0 + gp->by0, LX = TX + gp->bx1 - gp->bx0, LY = TY + gp->by1 - gp->by0;
-  int nm = gp->j1 - gp->j0 + gp->pad1, p0 = gp->pad0;
-  asm volatile("" : "+v"(bx), "+v"(by), "+v"(LX), "+v"(LY), "+v"(nm), "+v"(p0));
-  const int n = LX * LY;
-  BoxWalk wk(LY);
-#pragma unroll
-  for (int u = 0; u < kPfBox; ++u) {
-    const int k = threadIdx.x + u * blockDim.x;
-    const int kx = bx + wk.lx, ky = by + wk.ly;
-    pf.box[u] = make_uint4(0u, 0u, 0u, 0u);
-    if (k < n && (unsigned)kx < (unsigned)nx && (unsigned)ky < (unsigned)ny)
-      pf.box[u] = fsm_gload(ev.planes_h + (long)kx * ny + ky);
-    wk.next();
-  }
-  // the group's block (G.pad0): its offsets' records and B operands, then
-  // its G.pad1 pair MFMAs'
-  const int nw = nm * FSM_BMAT_U4;
-#pragma unroll
-  for (int u = 0; u < kPfW; ++u) {
-    const int k = threadIdx.x + u * blockDim.x;
-    if (k < nw) pf.w[u] = fsm_gload(ev.bmat + (long)p0 * FSM_BMAT_U4 + k);
-  }
-  if ((int)threadIdx.x < nm) {
-    const uint4 r = fsm_gload(reinterpret_cast<const uint4 *>(ev.moffs + p0 + threadIdx.x));
-    pf.rec = make_int4((int)r.x, (int)r.y, (int)r.z, (int)r.w);
-  }
-}
-
-__device__ __forceinline__ void group_store(const GroupPrefetch &pf, const Event2D &ev, const StencilGroup &G,
-                                            uint4 *lh, int x0, int y0, int TX, int TY) {
-  const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
-  const int no = G.j1 - G.j0 + G.pad1, n = LX * LY, step = blockDim.x;  // MFMAs of both forms
-  uint4 *bl = lh + LX * LYP;
-  int4 *orec = reinterpret_cast<int4 *>(bl + no * FSM_BMAT_U4);
-  BoxWalk wk(LY);
-#pragma unroll
-  for (int u = 0; u < kPfBox; ++u) {
-    if ((int)threadIdx.x + u * step < n) lh[wk.lx * LYP + wk.ly] = pf.box[u];
-    wk.next();
-  }
-  for (int k = threadIdx.x + kPfBox * step; k < n; k += step) {  // beyond the registers
-    const int lx = k / LY, ly = k - lx * LY;
-    const int kx = x0 + G.bx0 + lx, ky = y0 + G.by0 + ly;
-    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if ((unsigned)kx < (unsigned)ev.nx && (unsigned)ky < (unsigned)ev.ny) v = fsm_gload(ev.planes_h + (long)kx * ev.ny + ky);
-    lh[lx * LYP + ly] = v;
-  }
-  const int nw = no * FSM_BMAT_U4;
-#pragma unroll
-  for (int u = 0; u < kPfW; ++u) {
-    const int k = threadIdx.x + u * step;
-    if (k < nw) bl[k] = pf.w[u];
-  }
+  uint4 *bl = lh + n;
   const uint4 *bsrc = ev.bmat + (long)G.pad0 * FSM_BMAT_U4;
-  for (int k0 = threadIdx.x + kPfW * step; k0 < nw; k0 += 4 * step) {
-    uint4 v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + u * step < nw) v[u] = fsm_gload(bsrc + k0 + u * step);
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (k0 + u * step < nw) bl[k0 + u * step] = v[u];
-  }
-  if ((int)threadIdx.x < no) orec[threadIdx.x] = pf.rec;
+  for (int c = wave; c * 64 < nw; c += 4)
+    if (c * 64 + lane < nw) fsm_lds_dma16(bsrc + c * 64 + lane, bl + c * 64);
+  uint4 *rl = bl + nw;
+  const uint4 *rsrc = reinterpret_cast<const uint4 *>(ev.moffs + G.pad0);
+  for (int c = wave; c * 64 < no; c += 4)
+    if (c * 64 + lane < no) fsm_lds_dma16(rsrc + c * 64 + lane, rl + c * 64);
 }
 
 // MFMA tile shapes and the FP32 tile of the same width they fall back to.
@@ -584,8 +528,6 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) acc[xc][yb] = fsm_float4v{0.f, 0.f, 0.f, 0.f};
   const int ylast = min(y0 + TY - 1, ny - 1);
-  GroupPrefetch pf;
-  if (ev.n_mgroups > 0) group_load(pf, ev, ev.mgroups, x0, y0, TX, TY);
   for (int gi = 0; gi < ev.n_mgroups; ++gi) {
     const StencilGroup G = ev.mgroups[gi];
     // box columns are padded to 16 records (256 B) so that the lanes reading
@@ -594,11 +536,10 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     const int no = G.j1 - G.j0, np = G.pad1;
     uint4 *bl = lh + LX * LYP;  // [no + np][FSM_BMAT_U4] packed weights: offsets, then pair MFMAs
     int4 *orec = reinterpret_cast<int4 *>(bl + (no + np) * FSM_BMAT_U4);  // offsets (ox, oy, box delta, -), pairs (delta gy=0, gy=1, -, -)
+    __syncthreads();  // every wave is done with the previous group's image
+    group_dma(ev, G, lh, x0, y0, TX, TY, wave, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's copies have landed in LDS
     __syncthreads();
-    group_store(pf, ev, G, lh, x0, y0, TX, TY);
-    __syncthreads();
-    // next group's loads fly while this group's MFMAs run
-    if (gi + 1 < ev.n_mgroups) group_load(pf, ev, ev.mgroups + gi + 1, x0, y0, TX, TY);
     // every real cell's floor index stays inside [1, n-2] for all offsets of the group
     // lane's knot (corner) for cell column wave*XW, row r, at box offset 0
     const int lane_base = (wave * XW + gx) * LYP + r + gy;
@@ -606,10 +547,14 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     const int xw0 = x0 + wave * XW, xw1 = min(xw0 + XW - 1, nx - 1);
     const bool interior = (xw0 + G.bx0 >= 1) && (xw1 + G.bx1 - 1 <= nx - 2) && (y0 + G.by0 >= 1) &&
                           (ylast + G.by1 - 1 <= ny - 2);
-    if (interior)
-      mfma_pairs<XW>(acc, lh, bl + no * FSM_BMAT_U4, orec + no, np, lane_base - gy, LYP, gx, gy);
-    else
-      mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
+    // waves whose columns hold no live cell (the last tile column) skip the group
+    if (xw0 < nx) {
+      if (interior)
+        mfma_pairs<XW>(acc, lh, bl + no * FSM_BMAT_U4, orec + no, np, lane_base - gy, LYP, gx, gy,
+                       y0 + 16 < ny ? 2 : 1);
+      else
+        mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
+    }
   }
   // moments to LDS: Tt[m][lx*TY + ly]
   __syncthreads();
@@ -724,9 +669,9 @@ __global__ __launch_bounds__(256) void k_stream2d_batch(const Event2D *__restric
 }
 
 // Batched MFMA form (every 2-D event of a batch runs the context's path).
-// Three workgroups per CU: the register budget is capped at 168 per lane
-// (a few staging values spill; the MFMA loop does not), which measured 19%
-// faster than two workgroups per CU with the registers unconstrained.
+// Three workgroups per CU (the LDS image of a group is up to 48 KiB; the
+// kernel needs 142 registers per lane, under the cap of 168 that three
+// workgroups leave); two per CU measured 28% slower.
 __global__ __launch_bounds__(256, 3) void k_stream2d_mfma_batch(const Event2D *__restrict__ evs,
                                                              const int4 *__restrict__ items, int n_items) {
   extern __shared__ float4 lds4[];
