@@ -364,6 +364,14 @@ def main():
     value = mixed["events_per_s"]
     ms_per_step = mixed["ms_per_step"]
     info = ctx.plan_info(q1)  # context is on the mixed path here
+    # the drop-in call on host buffers (fsm_run: input copied in, 17 double
+    # outputs copied back each call), one event at a time: the PCIe-inclusive rate
+    ctx.run(q1, e1)
+    host_n = max(4, args.steps // 2)
+    t_host = timed(lambda: ctx.run(q1, e1), host_n, sync)
+    host_buffers = {"events_per_s": round(host_n / t_host, 2), "ms_per_event": round(t_host * 1e3 / host_n, 4),
+                    "bytes_per_event": D * (4 + 17 * 8), "path": "mixed",
+                    "note": "fsm_run on pageable host buffers, one C1 event per call (4 B in, 136 B out per cell)"}
     samples = D * q1.DIM_PHIP * q1.DIM_RAP * B
     k_ms = mixed["stream_launch_ms"]
     # algorithmic work: the reference's 52 FLOP per (cell, phi) sample (16
@@ -568,6 +576,7 @@ def main():
         "plain_mixed_events_per_s": round(mixed["plain_events_per_s"], 3),
         "mixed_batched_over_plain_fp32": round(value / fp32["plain_events_per_s"], 3),
         "mixed_vs_fp32_max_e_diff_rel": mixed_vs_fp32_e,
+        "host_buffers_c1": host_buffers,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4), "path": "mixed"},
         "stage_ms_per_event": {"mixed": {k: round(v, 5) for k, v in mixed["stage_ms_per_event"].items()},

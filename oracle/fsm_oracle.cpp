@@ -34,7 +34,15 @@
 // Parity pins for each part are listed in the tests that import this file
 // (tests/test_oracle_*.py): scipy natural splines, numpy.linalg.eig, the
 // Bessel closed forms of 2-D streaming of a Gaussian, the DTAU = 0 and
-// uniform-density known answers, and the pi.u = 0 / pi^mu_mu = 0 invariants.
+// uniform-density known answers, the pi.u = 0 / pi^mu_mu = 0 invariants, and
+// (tests/test_oracle_pins.py) closed forms for 3-D streaming and the 3-D
+// charge pass of affine profiles (trilinear interpolation is exact there),
+// the 3-D charge pass at DTAU = 0, E_DEP_FS on a two-valued profile and the
+// EOS_TYPE 2 fit against a double polyval of its coefficients.  Each of those
+// pins is shown to fail on a planted mistake: oracle_set_mutation() switches
+// one term of this file to a plausible wrong form (a trilinear corner, the
+// sign of the eta shift, the Jacobian, a rapidity factor, a norm, ...); the
+// product never calls it.
 
 #include <math.h>
 #include <stdio.h>
