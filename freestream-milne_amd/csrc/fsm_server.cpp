@@ -12,9 +12,13 @@
 // the event's plan) stays within max_batch_us and their number within
 // max_batch_events, makes the slot's stream wait for their uploads, and
 // launches them as one batch (one launch per stage); it retires slots whose
-// completion event has fired.  With two or more slots, slot 0 takes only 2-D
-// requests, so a 2-D request never waits behind a launch of 3-D events (each
-// of which is about a millisecond of device time).  Requests that arrive
+// completion event has fired.  With two or more slots, every slot but the
+// last two (at least one) takes only 2-D requests, so a 2-D request never
+// waits behind a launch of 3-D events (each of which is about a millisecond of
+// device time) and at most two such launches share the GPU with the 2-D
+// work: a 3-D event fills the GPU by itself, so a third would only take CU
+// slots from 2-D launches, while the second one's workgroups fill the first
+// one's tail.  Requests that arrive
 // while batches run are therefore admitted and launched next to the running
 // work.  Results stay in HBM until fetched or released.
 #include <hip/hip_runtime.h>
@@ -182,7 +186,7 @@ void worker_loop(fsm_server *s) {
     std::vector<Request *> take;
     for (size_t si = 0; si < s->slots.size() && take.empty() && !s->queued.empty(); ++si) {
       if (s->slots[si].busy) continue;
-      const bool small_only = s->slots.size() >= 2 && si == 0;
+      const bool small_only = s->slots.size() >= 2 && (int)si < std::max(1, (int)s->slots.size() - 2);
       std::deque<int64_t> skipped;  // stay queued, in order
       float work = 0.0f;
       while (!s->queued.empty()) {
@@ -257,7 +261,7 @@ int fsm_server_create(fsm_ctx *ctx, const fsm_server_config *cfg, fsm_server **o
   if (cfg) s->cfg = *cfg;
   if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = (size_t)16 << 30;
   if (s->cfg.max_batch_events <= 0) s->cfg.max_batch_events = 256;
-  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 3;
+  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 5;
   if (s->cfg.max_requests <= 0) s->cfg.max_requests = 4096;
   if (!(s->cfg.max_batch_us > 0.0f)) s->cfg.max_batch_us = 300.0f;
   int st = FSM_OK;

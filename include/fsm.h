@@ -338,7 +338,8 @@ typedef struct fsm_server fsm_server;
 typedef struct fsm_server_config {
   size_t hbm_budget_bytes; /* device memory for admitted requests (0: 16 GiB) */
   int max_batch_events;    /* events per launch (0: 256) */
-  int max_inflight;        /* launches in flight (0: 3) */
+  int max_inflight;        /* launches in flight (0: 5); with two or more, all but the last two
+                              (at least one) take only 2-D requests */
   int max_requests;        /* admitted + completed-unreleased requests (0: 4096) */
   float max_batch_us;      /* estimated device time per launch, microseconds (0: 300); a
                               launch always takes at least one request */

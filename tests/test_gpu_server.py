@@ -136,6 +136,39 @@ def test_2d_request_does_not_wait_behind_3d_launches(gpu_ctx):
     srv.close()
 
 
+def test_at_most_two_3d_launches_in_flight(gpu_ctx):
+    """With five launch slots, three take only 2-D requests: four 3-D requests
+    submitted together run two at a time (the third completes only after one
+    of the first two), and a 2-D request submitted after them completes before
+    the last two."""
+    import torch
+
+    p, e = W.config("C1")
+    p3, e3 = W.c2_like(41, seed=6)
+    srv = fsm_amd.Server(gpu_ctx, max_inflight=5)
+    s = torch.cuda.current_stream()
+    d2, d3 = torch.from_numpy(e).cuda(), torch.from_numpy(e3).cuda()
+    warm = [srv.submit_device(p3, d3.data_ptr(), e3.size, stream=s), srv.submit_device(p, d2.data_ptr(), e.size, stream=s)]
+    for _ in warm:
+        srv.release(srv.poll(5000000))
+    keep = _hold_stream(s)
+    four = [srv.submit_device(p3, d3.data_ptr(), e3.size, stream=s) for _ in range(4)]
+    two = srv.submit_device(p, d2.data_ptr(), e.size, stream=s)
+    order = []
+    while len(order) < 5:
+        rid = srv.poll(2000000)
+        assert rid is not None
+        order.append(rid)
+    pos = {rid: k for k, rid in enumerate(order)}
+    assert pos[four[2]] > min(pos[four[0]], pos[four[1]]), order
+    assert pos[four[3]] > min(pos[four[0]], pos[four[1]]), order
+    assert pos[two] < max(pos[four[2]], pos[four[3]]), order
+    for rid in order:
+        srv.release(rid)
+    del keep
+    srv.close()
+
+
 def test_budget_admission_and_cancel(gpu_ctx):
     p, e = W.config("C1")
     D = p.DIM_X * p.DIM_Y
