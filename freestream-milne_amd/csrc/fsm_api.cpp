@@ -2135,8 +2135,9 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     if (em && p.DIM_ETA > 1) {
       // 3-D: eta x transverse tiles of the LDS-transposed Landau form
       const int nxy = p.DIM_X * p.DIM_Y;
-      for (int et = 0; et < (p.DIM_ETA + FSML_TE - 1) / FSML_TE; ++et)
-        for (int xt = 0; xt < (nxy + FSML_TX - 1) / FSML_TX; ++xt) ld.push_back(make_int4(li, xt, et, 1));
+      // eta tiles fastest, as in the one-event launch (contiguous output stretches)
+      for (int xt = 0; xt < (nxy + FSML_TX - 1) / FSML_TX; ++xt)
+        for (int et = 0; et < (p.DIM_ETA + FSML_TE - 1) / FSML_TE; ++et) ld.push_back(make_int4(li, xt, et, 1));
       b->lds_l = std::max(b->lds_l, fsm_landau_lds_bytes(ev.d_nb != nullptr));
     } else {
       for (int c = 0; c < p.DIM; c += kLandauChunk)

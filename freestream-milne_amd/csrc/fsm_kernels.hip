@@ -1279,9 +1279,12 @@ __global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
 __global__ __launch_bounds__(256) void k_landau3d_one(EventLandau ev) {
   extern __shared__ float lds_out[];
   __shared__ LandauShared S;
-  const int xtiles = (ev.nxy + FSML_TX - 1) / FSML_TX;
+  // eta tiles fastest: the workgroups running together cover whole eta runs
+  // of a few transverse cells, i.e. contiguous stretches of every output
+  // plane (eta is the fast index of the reference layout)
+  const int etiles = (ev.neta + FSML_TE - 1) / FSML_TE;
   const int b = xcd_order(blockIdx.x, gridDim.x);
-  landau3d_tile(ev, b % xtiles, b / xtiles, lds_out, S);
+  landau3d_tile(ev, b / etiles, b % etiles, lds_out, S);
 }
 
 // Batched: item = (event, cell offset, cell count, 0) for 2-D events, or
