@@ -541,8 +541,8 @@ __device__ __forceinline__ void stage_q_rows(float *q, const float *__restrict__
       const int gx = lo + r;
       in[u] = k0 + u * FSM3D_THREADS < n && (unsigned)gx < (unsigned)nx && c < ny;
       const int src = in[u] ? gx * ny + c : 0;
-      e0[u] = p0[src];
-      e1[u] = p1[src];
+      e0[u] = *(const FSM_AS_GLOBAL float *)(p0 + src);
+      e1[u] = *(const FSM_AS_GLOBAL float *)(p1 + src);
       r += dr;
       c += dc;
       if (c >= pitch) {
@@ -592,7 +592,7 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
     for (int v = 0; v < 10; ++v) T[m][v] = 0.0f;
 
   for (int k = 0; k < ev.nrap; ++k) {
-    const RapSlot S = ev.slots[(long)ieta * ev.nrap + k];
+    const RapSlot S = fsm_sload(ev.slots + (long)ieta * ev.nrap + k);
     if (S.ns == 0) continue;
     const float w0 = 1.0f - S.feta, w1 = S.feta;
     const float *p0 = ev.src + (long)S.eta0 * nxy;
@@ -604,9 +604,9 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
       for (int c = 0; c < 6; ++c) mo[m][c] = 0.0f;
     int wlo = 1, whi = 0;  // empty window
     const Samp3D *sp = ev.samp + S.s0;
-    const int2 rg = ev.srange[((long)ieta * ev.nrap + k) * ev.tiles + tile];
+    const int2 rg = fsm_sload(ev.srange + ((long)ieta * ev.nrap + k) * ev.tiles + tile);
     for (int j = rg.x; j < rg.y; ++j) {
-      const Samp3D P = sp[j];
+      const Samp3D P = fsm_sload(sp + j);
       const int a = max(row_lo, P.xlo), b = min(row_hi, P.xhi);
       if (a > b) continue;  // no kept cell in this tile's rows
       if (a + P.ox < wlo || b + P.ox + 1 > whi) {
@@ -614,8 +614,8 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
         // they fit W rows (uniform over the workgroup)
         int lo = a + P.ox, hi = b + P.ox + 1;
         for (int j2 = j + 1; j2 < rg.y; ++j2) {
-          const int2 xr = *reinterpret_cast<const int2 *>(&sp[j2].xlo);
-          const int ox2 = sp[j2].ox;
+          const int2 xr = fsm_sload(reinterpret_cast<const int2 *>(&sp[j2].xlo));
+          const int ox2 = fsm_sload(&sp[j2].ox);
           const int a2 = max(row_lo, xr.x), b2 = min(row_hi, xr.y);
           if (a2 > b2) continue;
           const int l2 = min(lo, a2 + ox2), h2 = max(hi, b2 + ox2 + 1);
@@ -686,7 +686,7 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
     if (c >= nxy) continue;
     const long idx = (long)ieta * nxy + c;
 #pragma unroll
-    for (int v = 0; v < 10; ++v) ev.T[v * P + idx] = T[m][v] * ev.d_phip;  // :105
+    for (int v = 0; v < 10; ++v) fsm_gstore(ev.T + v * P + idx, T[m][v] * ev.d_phip);  // :105
   }
 }
 
@@ -702,8 +702,10 @@ __global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D 
                                                                   int n_items) {
   extern __shared__ float qlds[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[it];  // the host orders items by decreasing work
-    const Event3D ev = evs[item.x];
+    // descriptors through scalar loads: pointers read from memory are generic
+    // and their loads would otherwise go through flat vector loads
+    const int4 item = fsm_sload(items + it);  // the host orders items by decreasing work
+    const Event3D ev = fsm_sload(evs + item.x);
     stream3d_tile(ev, item.y, item.z, qlds);
   }
 }
