@@ -148,6 +148,7 @@ struct Plan {
   DevBuf<Samp3D> samp;
   DevBuf<int> eta_order;
   DevBuf<int2> srange;
+  DevBuf<int4> win3;  // staged-row windows of the 3-D stream kernel
   std::vector<float> eta_work;  // kept cell-samples of each eta plane (work estimate)
   // (cell, sample) pairs the reference's masks keep, and (3-D) the pairs the
   // stream kernel evaluates (whole tiles per sample whose rows it reaches)
@@ -681,10 +682,11 @@ void kept_interval(int n, int g0, float shift, float mn, float h, int *lo, int *
 }
 
 // One azimuth sample (cos c, sin s) of a slot with shift magnitude D:
-// offsets, bilinear weights and the kept intervals.  Returns -1 if a kept
-// cell would read outside [0, N] (cannot happen for grids of sane size; the
-// planner refuses the shape rather than read out of bounds).
-int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P) {
+// offsets, bilinear weights and the kept intervals.  Returns 0 with
+// *kept = false when the sample keeps no cell, -1 if a kept cell would read
+// outside [0, N] (cannot happen for grids of sane size; the planner refuses
+// the shape rather than read out of bounds).
+int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P, bool *kept) {
 #pragma clang fp contract(off)
   const int nx = p.DIM_X, ny = p.DIM_Y;
   const float xmin = (float)((-1.0) * ((double)(float)(nx - 1) / 2.0) * (double)p.DX);
@@ -694,17 +696,21 @@ int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P) {
   const float sx = -dxj * (1.0f / p.DX), sy = -dyj * (1.0f / p.DY);
   const float fx = floorf(sx), fy = floorf(sy);
   memset(P, 0, sizeof(*P));
-  P->xlo = P->ylo = 1;
-  P->xhi = P->yhi = 0;
+  *kept = false;
   // a sample shifted by more than the grid keeps nothing
   if (!(fabsf(sx) < (float)(nx + 2)) || !(fabsf(sy) < (float)(ny + 2))) return 0;
   P->ox = (int)fx;
   P->oy = (int)fy;
   const float tx = sx - fx, ty = sy - fy;
+  int ylo = 1, yhi = 0;
   kept_interval(nx, P->ox, dxj, xmin, p.DX, &P->xlo, &P->xhi);
-  kept_interval(ny, P->oy, dyj, ymin, p.DY, &P->ylo, &P->yhi);
-  if (P->xlo > P->xhi || P->ylo > P->yhi) return 0;
-  if (P->xlo + P->ox < 0 || P->xhi + P->ox + 1 > nx || P->ylo + P->oy < 0 || P->yhi + P->oy + 1 > ny) return -1;
+  kept_interval(ny, P->oy, dyj, ymin, p.DY, &ylo, &yhi);
+  if (P->xlo > P->xhi || ylo > yhi) return 0;
+  if (P->xlo + P->ox < 0 || P->xhi + P->ox + 1 > nx || ylo + P->oy < 0 || yhi + P->oy + 1 > ny) return -1;
+  *kept = true;
+  P->mlo = ((unsigned)P->xlo << 16) | (unsigned)ylo;
+  P->mext = ((unsigned)(P->xhi - P->xlo) << 16) | (unsigned)(yhi - ylo);
+  P->d0 = 4 * (P->ox * (ny + 1) + P->oy);
   P->w00 = (1.0f - tx) * (1.0f - ty);
   P->w01 = (1.0f - tx) * ty;
   P->w10 = tx * (1.0f - ty);
@@ -715,6 +721,45 @@ int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P) {
   P->cs = c * s;
   P->ss = s * s;
   return 0;
+}
+
+// Kept cell-samples of one 3-D sample (the product of its kept intervals).
+double samp3d_kept(const Samp3D &P) {
+  return (double)((P.mext >> 16) + 1) * (double)((P.mext & 0xFFFFu) + 1);
+}
+
+// Staged-row windows of one (slot, tile) pair for the 3-D stream kernel.
+// The tile is cells [c0, c0 + FSM3D_TILE) of the ix * NY + iy order, rows
+// [rlo, rhi].  Walking the slot's samples in their (ox, oy) order, a window
+// starts at the first sample that keeps a cell in the tile's rows and grows
+// over the following samples while the source rows their kept cells read fit
+// W rows; the sample that does not fit starts the next window.  Samples
+// keeping no cell of the tile's rows are left out (in the (ox, oy) order they
+// lie before or after the tile's samples).  Each window is (first sample,
+// end sample, first source row, rows).
+void plan_windows3d(const Samp3D *sp, int ns, int rlo, int rhi, int W, std::vector<int4> &out) {
+  int j = 0;
+  while (j < ns) {
+    const Samp3D &P = sp[j];
+    const int a = std::max(rlo, P.xlo), b = std::min(rhi, P.xhi);
+    if (a > b) {
+      ++j;
+      continue;
+    }
+    int lo = a + P.ox, hi = b + P.ox + 1, end = j + 1, j2 = j + 1;
+    for (; j2 < ns; ++j2) {
+      const Samp3D &Q = sp[j2];
+      const int a2 = std::max(rlo, Q.xlo), b2 = std::min(rhi, Q.xhi);
+      if (a2 > b2) continue;
+      const int l2 = std::min(lo, a2 + Q.ox), h2 = std::max(hi, b2 + Q.ox + 1);
+      if (h2 - l2 + 1 > W) break;
+      lo = l2;
+      hi = h2;
+      end = j2 + 1;
+    }
+    out.push_back(make_int4(j, end, lo, hi - lo + 1));
+    j = j2;
+  }
 }
 
 int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
@@ -772,8 +817,9 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
         S.s0 = (int)samp.size();
         for (int j = 0; j < nphi; ++j) {
           Samp3D P;
-          if (make_samp3d(cs[2 * j], cs[2 * j + 1], D, p, &P) < 0) return FSM_ERR_UNSUPPORTED;
-          if (P.xlo <= P.xhi && P.ylo <= P.yhi) samp.push_back(P);
+          bool kept = false;
+          if (make_samp3d(cs[2 * j], cs[2 * j + 1], D, p, &P, &kept) < 0) return FSM_ERR_UNSUPPORTED;
+          if (kept) samp.push_back(P);
         }
         S.ns = (int)samp.size() - S.s0;
         // by source row offset: a tile's samples form one index range and
@@ -794,7 +840,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     for (int k = 0; k < nrap; ++k) {
       const RapSlot &S = slots[(size_t)ie * nrap + k];
       for (int j = S.s0; j < S.s0 + S.ns; ++j) {
-        const double kept = (double)(samp[j].xhi - samp[j].xlo + 1) * (double)(samp[j].yhi - samp[j].ylo + 1);
+        const double kept = samp3d_kept(samp[j]);
         pl.eta_work[ie] += (float)kept;
         pl.samples_kept += kept;
       }
@@ -805,30 +851,38 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pl.eta_work[x] > pl.eta_work[y]; });
     FSM_TRY(pl.eta_order.upload(order));
   }
+  // staged-row window of the stream kernel: at least the rows one sample of a
+  // tile reads (the tile's rows + 1), else as many as fit FSM3D_WIN_FLOATS
+  int win_rows;
   {
-    // per (slot, tile): the index range of the slot's samples that keep a
-    // cell in the tile's rows (tiles are FSM3D_TILE consecutive cells)
+    const int need = (FSM3D_TILE - 1) / ny + 3;
+    win_rows = std::max(need, FSM3D_WIN_FLOATS / (ny + 1));
+    win_rows = std::min(win_rows, nx + 2);
+    win_rows = std::max(win_rows, need);
+  }
+  {
+    // per (slot, tile): the staged-row windows of the slot's samples that keep
+    // a cell in the tile's rows (tiles are FSM3D_TILE consecutive cells)
     const int tiles = (nx * ny + FSM3D_TILE - 1) / FSM3D_TILE;
     std::vector<int2> rg((size_t)neta * nrap * tiles, make_int2(0, 0));
+    std::vector<int4> wins;
     pl.samples_evaluated = 0.0;
     for (size_t si = 0; si < slots.size(); ++si) {
       const RapSlot &S = slots[si];
+      if (!S.valid || S.ns == 0) continue;
       for (int t = 0; t < tiles; ++t) {
         const int c0 = t * FSM3D_TILE;
         const int rlo = c0 / ny, rhi = std::min(c0 + FSM3D_TILE - 1, nx * ny - 1) / ny;
-        int first = -1, last = -1;
-        for (int j = 0; j < S.ns; ++j) {
-          const Samp3D &P = samp[S.s0 + j];
-          if (std::max(rlo, P.xlo) <= std::min(rhi, P.xhi)) {
-            if (first < 0) first = j;
-            last = j;
-            pl.samples_evaluated += FSM3D_TILE;  // every cell of the tile, dropped ones read zeros
-          }
-        }
-        if (first >= 0) rg[si * tiles + t] = make_int2(first, last + 1);
+        const size_t w0 = wins.size();
+        plan_windows3d(samp.data() + S.s0, S.ns, rlo, rhi, win_rows, wins);
+        for (size_t w = w0; w < wins.size(); ++w)
+          pl.samples_evaluated += (double)FSM3D_TILE * (wins[w].y - wins[w].x);  // whole tiles per sample
+        if (wins.size() > w0) rg[si * tiles + t] = make_int2((int)w0, (int)wins.size());
       }
     }
+    if (wins.empty()) wins.push_back(make_int4(0, 0, 0, 0));  // keep a valid pointer
     FSM_TRY(pl.srange.upload(rg));
+    FSM_TRY(pl.win3.upload(wins));
   }
   if (samp.empty()) samp.push_back(Samp3D{});  // keep a valid pointer
   FSM_TRY(pl.samp.upload(samp));
@@ -842,17 +896,9 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.nrap = nrap;
   ev.nphi = nphi;
   ev.tiles = (nx * ny + FSM3D_TILE - 1) / FSM3D_TILE;
-  {
-    // staged-row window: at least the rows one sample of a tile reads (the
-    // tile's rows + 1), else as many as fit FSM3D_WIN_FLOATS
-    const int pitch = ny + 1;
-    const int need = (FSM3D_TILE - 1) / ny + 3;
-    ev.win_rows = std::max(need, FSM3D_WIN_FLOATS / pitch);
-    ev.win_rows = std::min(ev.win_rows, nx + 2);
-    ev.win_rows = std::max(ev.win_rows, need);
-    ev.lds_floats = (ev.win_rows + 2) * pitch;  // + the 2 x 2 zero patch read by dropped cells
-    if ((size_t)ev.lds_floats * sizeof(float) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
-  }
+  ev.win_rows = win_rows;
+  ev.lds_floats = (win_rows + 2) * (ny + 1);  // + the 2 x 2 zero patch read by dropped cells
+  if ((size_t)ev.lds_floats * sizeof(float) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
   ev.d_phip = (float)((2.0 * M_PI) / (double)(float)nphi);
   ev.inv_dx = inv_dx;
   ev.inv_dy = inv_dy;
@@ -864,6 +910,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.slots = pl.slots.p;
   ev.samp = pl.samp.p;
   ev.srange = pl.srange.p;
+  ev.win = pl.win3.p;
   ev.eta_order = pl.eta_order.p;
   ev.T = pl.Tem.p;
   ev.t_plane = p.DIM;

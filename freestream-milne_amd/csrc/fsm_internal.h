@@ -260,20 +260,25 @@ struct RapSlot {
 // reference keeps a sample iff the floor of its float grid index lies in
 // [1, N-2] (src/FreeStream.cpp:213); that floor is monotone in the cell
 // index, so the kept cells form one interval per axis, found on the host with
-// the reference's own float expressions.
+// the reference's own float expressions.  The kernel reads the fields it
+// needs per sample already combined: the kept box packed for a 16-bit-lane
+// test against (ix << 16 | iy), and the LDS byte offset of the sample's
+// source corner relative to the cell in the staged-row window.
 struct Samp3D {
-  int ox, oy;                // floor offset (cells) of the shifted point
-  int xlo, xhi, ylo, yhi;    // kept cells: ix in [xlo, xhi], iy in [ylo, yhi]
+  unsigned mlo;              // (xlo << 16) | ylo: first kept cell along each axis
+  unsigned mext;             // ((xhi - xlo) << 16) | (yhi - ylo): kept extent - 1 along each axis
+  int d0;                    // 4 * (ox * (DIM_Y + 1) + oy): byte offset in a window of pitch DIM_Y + 1
   float w00, w01, w10, w11;  // bilinear weights of Q[ix+ox][iy+oy], [..][iy+oy+1], [ix+ox+1][..], [..+1][..+1]
   float c, s, cc, cs, ss;    // cos phi_j, sin phi_j, c*c, c*s, s*s
-  float pad;
+  int ox, oy;                // floor offset (cells) of the shifted point
+  int xlo, xhi;              // kept cells along x: ix in [xlo, xhi] (host window planning)
 };
 static_assert(sizeof(Samp3D) == 64, "Samp3D is read with one scalar load");
 
 struct Event3D {
   int nx, ny, neta, nrap, nphi;
   int tiles;            // transverse tiles per eta plane
-  int lds_floats;       // LDS floats of the staged-row window (win_rows * (NY+1))
+  int lds_floats;       // LDS floats of the staged-row window and its zero patch ((win_rows + 2) * (NY + 1))
   int win_rows;         // rows of the window
   float d_phip;
   float inv_dx, inv_dy; // 1/DX, 1/DY: shift in cells = -(cos(phi) * D) * inv_dx
@@ -283,8 +288,9 @@ struct Event3D {
   float *src;           // eta-major e0*norm planes
   const RapSlot *slots; // [neta][nrap]
   const Samp3D *samp;   // live samples of every slot, each slot's sorted by (ox, oy)
-  const int2 *srange;   // [neta*nrap][tiles]: samples [x, y) of the slot (relative to s0) that keep a
-                        // cell in the tile's rows
+  const int2 *srange;   // [neta*nrap][tiles]: the slot's staged-row windows [x, y) for the tile
+  const int4 *win;      // windows: samples [x, y) of the slot (relative to s0), served by window rows
+                        // [z, z + w) of the eta-blended source plane
   const int *eta_order; // eta planes by decreasing work (the one-event launch's order)
   float *T;             // 10 planes, eta-major layout
   long t_plane;

@@ -509,16 +509,16 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ in,
 // One transverse tile (FSM3D_TILE consecutive cells of the ix*NY+iy order,
 // i.e. a few whole rows) of one eta plane.  Per rapidity slot the kernel walks
 // the slot's live azimuth samples (host tables: offset, bilinear weights and
-// the interval of cells the reference keeps along each axis).  The source the
-// samples read is the eta-blended plane
+// the box of cells the reference keeps).  The source the samples read is the
+// eta-blended plane
 //   Q(x, y) = (1-f) rf0 E[eta0](x, y) + f rf1 E[eta0+1](x, y),
 // staged in LDS as a window of whole rows (pitch NY+1: column NY and any row
-// outside the grid are zero).  A window covers the rows a run of consecutive
-// samples needs for this tile and is restaged when the next sample falls
-// outside it, so a slot whose samples only graze the grid stages only what
-// they read.  Every read lies in the window: a kept cell's source rows
-// ix+ox, ix+ox+1 lie in [0, NX] and columns iy+oy, iy+oy+1 in [0, NY]
-// (checked on the host).
+// outside the grid are zero).  The host plans the windows of every (slot,
+// tile) pair: a window holds the source rows that a run of consecutive
+// samples reads for this tile, so a slot whose samples only graze the grid
+// stages only what they read.  Every read lies in the window: a kept cell's
+// source rows ix+ox, ix+ox+1 lie in the window's rows and columns
+// iy+oy, iy+oy+1 in [0, NY] (checked on the host).
 // Stages window rows [lo, lo + nr) of Q(x, y) = w0 (E0(x, y) rf0) + w1 (E1(x, y) rf1)
 // (pitch ny + 1; rows outside the grid and column ny are zero).  The rows
 // are walked as one flat range, eight elements per thread per round, with
@@ -554,7 +554,8 @@ __device__ __forceinline__ void stage_q_rows(float *q, const float *__restrict__
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u * FSM3D_THREADS;
       if (k >= n) break;
-      q[k] = in[u] ? w0 * (e0[u] * rf0) + w1 * (e1[u] * rf1) : 0.0f;
+      // one rounding order wherever the element falls in the unrolled loop
+      q[k] = in[u] ? fmaf(w0, e0[u] * rf0, w1 * (e1[u] * rf1)) : 0.0f;
     }
   }
 }
@@ -562,17 +563,57 @@ __device__ __forceinline__ void stage_q_rows(float *q, const float *__restrict__
 typedef unsigned short fsm_u16x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) const float fsm_lds_cf;  // 32-bit LDS addresses
 
+// Bilinear values of one sample at the lane's FSM3D_M cells: a cell outside
+// the sample's kept box (one packed 16-bit subtract/min/compare of
+// (ix << 16 | iy) against the box) reads the 2 x 2 zero patch after the
+// window, so its value needs no select.
+struct Q3Taps {
+  float q00[FSM3D_M], q01[FSM3D_M], q10[FSM3D_M], q11[FSM3D_M];
+};
+__device__ __forceinline__ void q3_taps(const Samp3D &P, unsigned wb, const unsigned (&pk)[FSM3D_M],
+                                        const unsigned (&ba)[FSM3D_M], unsigned zero_b, unsigned pitch_b,
+                                        Q3Taps &q) {
+  const unsigned delta_b = (unsigned)P.d0 - wb;  // modulo 2^32
+#pragma unroll
+  for (int m = 0; m < FSM3D_M; ++m) {
+    const fsm_u16x2 t = __builtin_bit_cast(fsm_u16x2, pk[m]) - __builtin_bit_cast(fsm_u16x2, P.mlo);
+    const fsm_u16x2 mn = __builtin_elementwise_min(t, __builtin_bit_cast(fsm_u16x2, P.mext));
+    const bool ok = __builtin_bit_cast(unsigned, mn) == __builtin_bit_cast(unsigned, t);
+    const unsigned ad = ok ? ba[m] + delta_b : zero_b;
+    fsm_lds_cf *qa = (fsm_lds_cf *)(size_t)ad;
+    fsm_lds_cf *qc = (fsm_lds_cf *)(size_t)(ad + pitch_b);
+    q.q00[m] = qa[0];
+    q.q01[m] = qa[1];
+    q.q10[m] = qc[0];
+    q.q11[m] = qc[1];
+  }
+}
+// Adds one sample's value at the lane's cells to the six angular moments.
+__device__ __forceinline__ void q3_moments(const Samp3D &P, const Q3Taps &q, float (&mo)[FSM3D_M][6]) {
+#pragma unroll
+  for (int m = 0; m < FSM3D_M; ++m) {
+    float v = q.q00[m] * P.w00;
+    v = fmaf(q.q01[m], P.w01, v);
+    v = fmaf(q.q10[m], P.w10, v);
+    v = fmaf(q.q11[m], P.w11, v);
+    mo[m][0] += v;
+    mo[m][1] = fmaf(v, P.c, mo[m][1]);
+    mo[m][2] = fmaf(v, P.s, mo[m][2]);
+    mo[m][3] = fmaf(v, P.cc, mo[m][3]);
+    mo[m][4] = fmaf(v, P.cs, mo[m][4]);
+    mo[m][5] = fmaf(v, P.ss, mo[m][5]);
+  }
+}
+
 __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int nx = ev.nx, ny = ev.ny, nxy = nx * ny;
   const int pitch = ny + 1;
   const int W = ev.win_rows;
   const int c0 = tile * FSM3D_TILE;
-  const int row_lo = c0 / ny;
-  const int row_hi = min(c0 + FSM3D_TILE - 1, nxy - 1) / ny;
   // the lane's cells c0 + tid + 256 m as (ix << 16 | iy) (all ones past the
-  // grid: fails every kept-interval test) and as LDS byte address in the
-  // window at zero row offset; a dropped cell reads the 2 x 2 zero patch
-  // after the window (rows W, W+1, columns 0, 1)
+  // grid: fails every kept-box test) and as LDS byte address in the window at
+  // zero row offset; a dropped cell reads the 2 x 2 zero patch after the
+  // window (rows W, W+1, columns 0, 1)
   const unsigned qbase = (unsigned)(size_t)(fsm_lds_cf *)q;
   unsigned pk[FSM3D_M];
   unsigned ba[FSM3D_M];
@@ -594,6 +635,8 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   for (int k = 0; k < ev.nrap; ++k) {
     const RapSlot S = fsm_sload(ev.slots + (long)ieta * ev.nrap + k);
     if (S.ns == 0) continue;
+    const int2 wr = fsm_sload(ev.srange + ((long)ieta * ev.nrap + k) * ev.tiles + tile);
+    if (wr.x >= wr.y) continue;  // no sample keeps a cell of this tile
     const float w0 = 1.0f - S.feta, w1 = S.feta;
     const float *p0 = ev.src + (long)S.eta0 * nxy;
     const float *p1 = ev.src + (long)(S.eta0 + 1) * nxy;
@@ -602,65 +645,31 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
     for (int m = 0; m < FSM3D_M; ++m)
 #pragma unroll
       for (int c = 0; c < 6; ++c) mo[m][c] = 0.0f;
-    int wlo = 1, whi = 0;  // empty window
     const Samp3D *sp = ev.samp + S.s0;
-    const int2 rg = fsm_sload(ev.srange + ((long)ieta * ev.nrap + k) * ev.tiles + tile);
-    for (int j = rg.x; j < rg.y; ++j) {
-      const Samp3D P = fsm_sload(sp + j);
-      const int a = max(row_lo, P.xlo), b = min(row_hi, P.xhi);
-      if (a > b) continue;  // no kept cell in this tile's rows
-      if (a + P.ox < wlo || b + P.ox + 1 > whi) {
-        // new window: the rows of this and the following samples while
-        // they fit W rows (uniform over the workgroup)
-        int lo = a + P.ox, hi = b + P.ox + 1;
-        for (int j2 = j + 1; j2 < rg.y; ++j2) {
-          const int2 xr = fsm_sload(reinterpret_cast<const int2 *>(&sp[j2].xlo));
-          const int ox2 = fsm_sload(&sp[j2].ox);
-          const int a2 = max(row_lo, xr.x), b2 = min(row_hi, xr.y);
-          if (a2 > b2) continue;
-          const int l2 = min(lo, a2 + ox2), h2 = max(hi, b2 + ox2 + 1);
-          if (h2 - l2 + 1 > W) break;
-          lo = l2;
-          hi = h2;
-        }
-        __syncthreads();  // previous window no longer read
-        wlo = lo;
-        whi = hi;
-        stage_q_rows(q, p0, p1, w0, w1, S.rf0, S.rf1, lo, hi - lo + 1, nx, ny);
-        if (threadIdx.x < 4) q[W * pitch + (threadIdx.x & 1) + (threadIdx.x >> 1) * pitch] = 0.0f;
-        __syncthreads();
+    for (int w = wr.x; w < wr.y; ++w) {
+      const int4 win = fsm_sload(ev.win + w);  // samples [x, y), source rows [z, z + w)
+      __syncthreads();  // previous window no longer read
+      stage_q_rows(q, p0, p1, w0, w1, S.rf0, S.rf1, win.z, win.w, nx, ny);
+      if (threadIdx.x < 4) q[W * pitch + (threadIdx.x & 1) + (threadIdx.x >> 1) * pitch] = 0.0f;
+      __syncthreads();
+      const unsigned wb = 4u * (unsigned)(win.z * pitch);
+      // two samples per step: the second one's LDS reads are in flight while
+      // the first one's moments are summed (the order of the sums is kept)
+      int j = win.x;
+      for (; j + 1 < win.y; j += 2) {
+        const Samp3D P0 = fsm_sload(sp + j);
+        const Samp3D P1 = fsm_sload(sp + j + 1);
+        Q3Taps qa, qb;
+        q3_taps(P0, wb, pk, ba, zero_b, pitch_b, qa);
+        q3_taps(P1, wb, pk, ba, zero_b, pitch_b, qb);
+        q3_moments(P0, qa, mo);
+        q3_moments(P1, qb, mo);
       }
-      // branch-free over the lane's cells: a dropped cell reads the zero patch
-      // (ix - xlo <= xhi - xlo and iy - ylo <= yhi - ylo, unsigned per 16-bit half)
-      const unsigned lo = ((unsigned)P.xlo << 16) | (unsigned)P.ylo;
-      const unsigned ext = ((unsigned)(P.xhi - P.xlo) << 16) | (unsigned)(P.yhi - P.ylo);
-      const unsigned delta_b = 4u * (unsigned)((P.ox - wlo) * pitch + P.oy);  // modulo 2^32
-      float q00[FSM3D_M], q01[FSM3D_M], q10[FSM3D_M], q11[FSM3D_M];
-#pragma unroll
-      for (int m = 0; m < FSM3D_M; ++m) {
-        const fsm_u16x2 t = __builtin_bit_cast(fsm_u16x2, pk[m]) - __builtin_bit_cast(fsm_u16x2, lo);
-        const fsm_u16x2 mn = __builtin_elementwise_min(t, __builtin_bit_cast(fsm_u16x2, ext));
-        const bool ok = __builtin_bit_cast(unsigned, mn) == __builtin_bit_cast(unsigned, t);
-        const unsigned ad = ok ? ba[m] + delta_b : zero_b;
-        fsm_lds_cf *qa = (fsm_lds_cf *)(size_t)ad;
-        fsm_lds_cf *qc = (fsm_lds_cf *)(size_t)(ad + pitch_b);
-        q00[m] = qa[0];
-        q01[m] = qa[1];
-        q10[m] = qc[0];
-        q11[m] = qc[1];
-      }
-#pragma unroll
-      for (int m = 0; m < FSM3D_M; ++m) {
-        float v = q00[m] * P.w00;
-        v = fmaf(q01[m], P.w01, v);
-        v = fmaf(q10[m], P.w10, v);
-        v = fmaf(q11[m], P.w11, v);
-        mo[m][0] += v;
-        mo[m][1] = fmaf(v, P.c, mo[m][1]);
-        mo[m][2] = fmaf(v, P.s, mo[m][2]);
-        mo[m][3] = fmaf(v, P.cc, mo[m][3]);
-        mo[m][4] = fmaf(v, P.cs, mo[m][4]);
-        mo[m][5] = fmaf(v, P.ss, mo[m][5]);
+      if (j < win.y) {
+        const Samp3D P0 = fsm_sload(sp + j);
+        Q3Taps qa;
+        q3_taps(P0, wb, pk, ba, zero_b, pitch_b, qa);
+        q3_moments(P0, qa, mo);
       }
     }
     // hypertrig rows (src/LandauMatch.cpp:46-55) times the Jacobian (:93,101)
@@ -690,16 +699,16 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   }
 }
 
-__global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_one(Event3D ev) {
+__global__ __launch_bounds__(FSM3D_THREADS, 4) void k_stream3d_one(Event3D ev) {
   extern __shared__ float qlds[];
   // eta planes in decreasing order of work, so the last workgroups to start
   // are the short ones; tiles of one plane round-robin over the XCDs
   stream3d_tile(ev, blockIdx.x % ev.tiles, ev.eta_order[blockIdx.x / ev.tiles], qlds);
 }
 
-__global__ __launch_bounds__(FSM3D_THREADS) void k_stream3d_batch(const Event3D *__restrict__ evs,
-                                                                  const int4 *__restrict__ items,
-                                                                  int n_items) {
+__global__ __launch_bounds__(FSM3D_THREADS, 4) void k_stream3d_batch(const Event3D *__restrict__ evs,
+                                                                     const int4 *__restrict__ items,
+                                                                     int n_items) {
   extern __shared__ float qlds[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     // descriptors through scalar loads: pointers read from memory are generic
@@ -1383,11 +1392,9 @@ hipError_t fsm_launch_landau(const EventLandau &ev, hipStream_t s) {
 hipError_t fsm_set_lds_limits(int bytes2d, int bytes3d) {
   hipError_t e = fsm_stream2d_set_lds_limit(bytes2d);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_stream3d_one, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          bytes3d);
+  e = hipFuncSetAttribute((const void *)k_stream3d_one, hipFuncAttributeMaxDynamicSharedMemorySize, bytes3d);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)k_stream3d_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             bytes3d);
+  return hipFuncSetAttribute((const void *)k_stream3d_batch, hipFuncAttributeMaxDynamicSharedMemorySize, bytes3d);
 }
 
 // ---- batched launchers: one launch per stage for every event of a batch ----
