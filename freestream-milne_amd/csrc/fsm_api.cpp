@@ -710,7 +710,7 @@ int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P, bool 
   *kept = true;
   P->mlo = ((unsigned)P->xlo << 16) | (unsigned)ylo;
   P->mext = ((unsigned)(P->xhi - P->xlo) << 16) | (unsigned)(yhi - ylo);
-  P->d0 = 4 * (P->ox * (ny + 1) + P->oy);
+  P->d0 = 4 * (P->ox * ny + P->oy);
   P->w00 = (1.0f - tx) * (1.0f - ty);
   P->w01 = (1.0f - tx) * ty;
   P->w10 = tx * (1.0f - ty);
@@ -719,7 +719,6 @@ int make_samp3d(float c, float s, float D, const fsm_params &p, Samp3D *P, bool 
   P->s = s;
   P->cc = c * c;
   P->cs = c * s;
-  P->ss = s * s;
   return 0;
 }
 
@@ -856,7 +855,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   int win_rows;
   {
     const int need = (FSM3D_TILE - 1) / ny + 3;
-    win_rows = std::max(need, FSM3D_WIN_FLOATS / (ny + 1));
+    win_rows = std::max(need, FSM3D_WIN_FLOATS / ny);
     win_rows = std::min(win_rows, nx + 2);
     win_rows = std::max(win_rows, need);
   }
@@ -897,7 +896,7 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
   ev.nphi = nphi;
   ev.tiles = (nx * ny + FSM3D_TILE - 1) / FSM3D_TILE;
   ev.win_rows = win_rows;
-  ev.lds_floats = (win_rows + 2) * (ny + 1);  // + the 2 x 2 zero patch read by dropped cells
+  ev.lds_floats = (win_rows + 2) * ny;  // + the 2 x 2 zero patch read by dropped cells
   if ((size_t)ev.lds_floats * sizeof(float) > (size_t)ctx->max_lds) return FSM_ERR_UNSUPPORTED;
   ev.d_phip = (float)((2.0 * M_PI) / (double)(float)nphi);
   ev.inv_dx = inv_dx;

@@ -267,18 +267,19 @@ struct RapSlot {
 struct Samp3D {
   unsigned mlo;              // (xlo << 16) | ylo: first kept cell along each axis
   unsigned mext;             // ((xhi - xlo) << 16) | (yhi - ylo): kept extent - 1 along each axis
-  int d0;                    // 4 * (ox * (DIM_Y + 1) + oy): byte offset in a window of pitch DIM_Y + 1
+  int d0;                    // 4 * (ox * DIM_Y + oy): byte offset in a window of pitch DIM_Y
   float w00, w01, w10, w11;  // bilinear weights of Q[ix+ox][iy+oy], [..][iy+oy+1], [ix+ox+1][..], [..+1][..+1]
-  float c, s, cc, cs, ss;    // cos phi_j, sin phi_j, c*c, c*s, s*s
+  float c, s, cc, cs;        // cos phi_j, sin phi_j, c*c, c*s
   int ox, oy;                // floor offset (cells) of the shifted point
   int xlo, xhi;              // kept cells along x: ix in [xlo, xhi] (host window planning)
+  int pad;
 };
 static_assert(sizeof(Samp3D) == 64, "Samp3D is read with one scalar load");
 
 struct Event3D {
   int nx, ny, neta, nrap, nphi;
   int tiles;            // transverse tiles per eta plane
-  int lds_floats;       // LDS floats of the staged-row window and its zero patch ((win_rows + 2) * (NY + 1))
+  int lds_floats;       // LDS floats of the staged-row window and its zero patch ((win_rows + 2) * NY)
   int win_rows;         // rows of the window
   float d_phip;
   float inv_dx, inv_dy; // 1/DX, 1/DY: shift in cells = -(cos(phi) * D) * inv_dx

@@ -512,52 +512,51 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ in,
 // the box of cells the reference keeps).  The source the samples read is the
 // eta-blended plane
 //   Q(x, y) = (1-f) rf0 E[eta0](x, y) + f rf1 E[eta0+1](x, y),
-// staged in LDS as a window of whole rows (pitch NY+1: column NY and any row
-// outside the grid are zero).  The host plans the windows of every (slot,
-// tile) pair: a window holds the source rows that a run of consecutive
-// samples reads for this tile, so a slot whose samples only graze the grid
-// stages only what they read.  Every read lies in the window: a kept cell's
-// source rows ix+ox, ix+ox+1 lie in the window's rows and columns
-// iy+oy, iy+oy+1 in [0, NY] (checked on the host).
-// Stages window rows [lo, lo + nr) of Q(x, y) = w0 (E0(x, y) rf0) + w1 (E1(x, y) rf1)
-// (pitch ny + 1; rows outside the grid and column ny are zero).  The rows
-// are walked as one flat range, eight elements per thread per round, with
-// every load issued before the first use (out-of-range elements load element
-// 0 and store 0), so a restage costs about one memory latency per round
-// rather than one per row.
+// staged in LDS as a window of whole rows (pitch NY, rows outside the grid
+// zero).  The host plans the windows of every (slot, tile) pair: a window
+// holds the source rows that a run of consecutive samples reads for this
+// tile, so a slot whose samples only graze the grid stages only what they
+// read.  Every read lies in the window: a kept cell's source rows ix+ox,
+// ix+ox+1 lie in the window's rows and its columns iy+oy, iy+oy+1 in
+// [1, NY-1] (the kept box; checked on the host).  With pitch NY consecutive
+// cells of the tile read consecutive LDS words across a row boundary too.
+//
+// Stages window rows [lo, lo + nr) of Q = a0 E0 + a1 E1 (a0 = (1-f) rf0,
+// a1 = f rf1).  Lane l of wave w covers columns l, l + 64, ... of rows w,
+// w + 4, ...; four rows' loads are issued before their first use.
 __device__ __forceinline__ void stage_q_rows(float *q, const float *__restrict__ p0, const float *__restrict__ p1,
-                                             float w0, float w1, float rf0, float rf1, int lo, int nr, int nx,
-                                             int ny) {
-  constexpr int U = 8;
-  const int pitch = ny + 1, n = nr * pitch;
-  // element k of the window is row k / pitch, column k % pitch, walked incrementally
-  const int dr = FSM3D_THREADS / pitch, dc = FSM3D_THREADS - dr * pitch;
-  int r = (int)threadIdx.x / pitch, c = (int)threadIdx.x - r * pitch;
-  for (int k0 = (int)threadIdx.x; k0 < n; k0 += U * FSM3D_THREADS) {
-    float e0[U], e1[U];
-    bool in[U];
+                                             float a0, float a1, int lo, int nr, int nx, int ny) {
+  constexpr int RU = 4;
+  const int lane = (int)threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  // byte offsets from a wave-uniform base: 32-bit address arithmetic
+  const FSM_AS_GLOBAL char *g0 = (const FSM_AS_GLOBAL char *)p0;
+  const FSM_AS_GLOBAL char *g1 = (const FSM_AS_GLOBAL char *)p1;
+  for (int c = lane; c - lane < ny; c += 64) {
+    const bool cin = c < ny;
+    for (int r0 = wave; r0 < nr; r0 += 4 * RU) {
+      float e0[RU], e1[RU];
+      bool ok[RU];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int gx = lo + r;
-      in[u] = k0 + u * FSM3D_THREADS < n && (unsigned)gx < (unsigned)nx && c < ny;
-      const int src = in[u] ? gx * ny + c : 0;
-      e0[u] = *(const FSM_AS_GLOBAL float *)(p0 + src);
-      e1[u] = *(const FSM_AS_GLOBAL float *)(p1 + src);
-      r += dr;
-      c += dc;
-      if (c >= pitch) {
-        c -= pitch;
-        ++r;
+      for (int u = 0; u < RU; ++u) {
+        const int gx = lo + r0 + 4 * u;  // wave-uniform
+        ok[u] = cin && r0 + 4 * u < nr && (unsigned)gx < (unsigned)nx;
+        // an element off the grid loads element 0 and stores 0
+        const unsigned ob = ok[u] ? 4u * (unsigned)(gx * ny + c) : 0u;
+        e0[u] = *(const FSM_AS_GLOBAL float *)(g0 + ob);
+        e1[u] = *(const FSM_AS_GLOBAL float *)(g1 + ob);
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int r = r0 + 4 * u;
+        if (cin && r < nr) q[r * ny + c] = ok[u] ? fmaf(a0, e0[u], a1 * e1[u]) : 0.0f;
       }
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u * FSM3D_THREADS;
-      if (k >= n) break;
-      // one rounding order wherever the element falls in the unrolled loop
-      q[k] = in[u] ? fmaf(w0, e0[u] * rf0, w1 * (e1[u] * rf1)) : 0.0f;
-    }
   }
+  // row nr is zero: a kept cell whose rounding-decided floor index sits one
+  // column left of the offset's may read column NY with weight 0, i.e. the
+  // next row's column 0 (a staged row, or this one)
+  for (int c = (int)threadIdx.x; c < ny; c += FSM3D_THREADS) q[nr * ny + c] = 0.0f;
 }
 
 typedef unsigned short fsm_u16x2 __attribute__((ext_vector_type(2)));
@@ -588,8 +587,9 @@ __device__ __forceinline__ void q3_taps(const Samp3D &P, unsigned wb, const unsi
     q.q11[m] = qc[1];
   }
 }
-// Adds one sample's value at the lane's cells to the six angular moments.
-__device__ __forceinline__ void q3_moments(const Samp3D &P, const Q3Taps &q, float (&mo)[FSM3D_M][6]) {
+// Adds one sample's value v at the lane's cells to the angular moments
+// v, v cos, v sin, v cos^2, v cos sin (v sin^2 = v - v cos^2 at the end).
+__device__ __forceinline__ void q3_moments(const Samp3D &P, const Q3Taps &q, float (&mo)[FSM3D_M][5]) {
 #pragma unroll
   for (int m = 0; m < FSM3D_M; ++m) {
     float v = q.q00[m] * P.w00;
@@ -601,13 +601,12 @@ __device__ __forceinline__ void q3_moments(const Samp3D &P, const Q3Taps &q, flo
     mo[m][2] = fmaf(v, P.s, mo[m][2]);
     mo[m][3] = fmaf(v, P.cc, mo[m][3]);
     mo[m][4] = fmaf(v, P.cs, mo[m][4]);
-    mo[m][5] = fmaf(v, P.ss, mo[m][5]);
   }
 }
 
 __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
   const int nx = ev.nx, ny = ev.ny, nxy = nx * ny;
-  const int pitch = ny + 1;
+  const int pitch = ny;
   const int W = ev.win_rows;
   const int c0 = tile * FSM3D_TILE;
   // the lane's cells c0 + tid + 256 m as (ix << 16 | iy) (all ones past the
@@ -637,19 +636,19 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
     if (S.ns == 0) continue;
     const int2 wr = fsm_sload(ev.srange + ((long)ieta * ev.nrap + k) * ev.tiles + tile);
     if (wr.x >= wr.y) continue;  // no sample keeps a cell of this tile
-    const float w0 = 1.0f - S.feta, w1 = S.feta;
+    const float a0 = (1.0f - S.feta) * S.rf0, a1 = S.feta * S.rf1;
     const float *p0 = ev.src + (long)S.eta0 * nxy;
     const float *p1 = ev.src + (long)(S.eta0 + 1) * nxy;
-    float mo[FSM3D_M][6];
+    float mo[FSM3D_M][5];
 #pragma unroll
     for (int m = 0; m < FSM3D_M; ++m)
 #pragma unroll
-      for (int c = 0; c < 6; ++c) mo[m][c] = 0.0f;
+      for (int c = 0; c < 5; ++c) mo[m][c] = 0.0f;
     const Samp3D *sp = ev.samp + S.s0;
     for (int w = wr.x; w < wr.y; ++w) {
       const int4 win = fsm_sload(ev.win + w);  // samples [x, y), source rows [z, z + w)
       __syncthreads();  // previous window no longer read
-      stage_q_rows(q, p0, p1, w0, w1, S.rf0, S.rf1, win.z, win.w, nx, ny);
+      stage_q_rows(q, p0, p1, a0, a1, win.z, win.w, nx, ny);
       if (threadIdx.x < 4) q[W * pitch + (threadIdx.x & 1) + (threadIdx.x >> 1) * pitch] = 0.0f;
       __syncthreads();
       const unsigned wb = 4u * (unsigned)(win.z * pitch);
@@ -683,7 +682,7 @@ __device__ void stream3d_tile(const Event3D &ev, int tile, int ieta, float *q) {
       T[m][4] = fmaf(j0 * S.inv_ch2, mo[m][3], T[m][4]);
       T[m][5] = fmaf(j0 * S.inv_ch2, mo[m][4], T[m][5]);
       T[m][6] = fmaf(j0 * S.th_tch, mo[m][1], T[m][6]);
-      T[m][7] = fmaf(j0 * S.inv_ch2, mo[m][5], T[m][7]);
+      T[m][7] = fmaf(j0 * S.inv_ch2, mo[m][0] - mo[m][3], T[m][7]);
       T[m][8] = fmaf(j0 * S.th_tch, mo[m][2], T[m][8]);
       T[m][9] = fmaf(j0 * S.th2_t2, mo[m][0], T[m][9]);
     }
