@@ -121,6 +121,7 @@ struct Plan {
   fsm_params key{};
   bool valid = false;
   bool is3d = false;
+  int rapidity_rule = FSM_RAPIDITY_TAN;  // 3-D plans: the rule their rapidity tables follow
   // 2-D
   Event2D ev2{};
   DevBuf<Stencil2D> st;
@@ -184,6 +185,70 @@ float charge_norm_factor(float SB) {
 
 bool same_shape(const fsm_params &a, const fsm_params &b) { return memcmp(&a, &b, sizeof(a)) == 0; }
 
+// Gauss-Hermite rule for the weight exp(-x^2) from the three-term recurrence
+// of the orthonormal Hermite polynomials
+//   p_0 = pi^(-1/4), p_{j+1} = x sqrt(2/(j+1)) p_j - sqrt(j/(j+1)) p_{j-1}:
+// the nodes are the zeros of p_n (all inside |x| < sqrt(2n+1)), bracketed by
+// sign changes on a grid finer than their spacing and refined by bisection;
+// the weights are the Christoffel numbers 1 / sum_{j<n} p_j(x)^2.
+void hermite_orthonormal(int n, double x, double *pn, double *sum_sq) {
+  double pm = 0.0, p = 0.7511255444649425;  // pi^(-1/4)
+  double ss = 0.0;
+  for (int j = 0; j < n; ++j) {
+    ss += p * p;
+    const double pj = x * sqrt(2.0 / (j + 1)) * p - sqrt((double)j / (j + 1)) * pm;
+    pm = p;
+    p = pj;
+  }
+  *pn = p;
+  *sum_sq = ss;
+}
+
+int gauss_hermite_rule(int n, double *x, double *w) {
+  if (n < 1 || n > 200 || !x || !w) return FSM_ERR_ARG;
+  const double L = sqrt(2.0 * n + 1.0) + 1.0;
+  const double h = 0.25 / sqrt(2.0 * n + 1.0);  // zero spacing is at least ~pi / sqrt(2n+1)
+  int found = 0;
+  double a = -L, fa, ss;
+  hermite_orthonormal(n, a, &fa, &ss);
+  while (a < L && found < n) {
+    const double b = std::min(a + h, L);
+    double fb;
+    hermite_orthonormal(n, b, &fb, &ss);
+    if (fb == 0.0 || (fa < 0.0) != (fb < 0.0)) {
+      double lo = a, hi = b, flo = fa;
+      for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, fabs(lo)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        double fm;
+        hermite_orthonormal(n, mid, &fm, &ss);
+        if ((fm < 0.0) == (flo < 0.0)) {
+          lo = mid;
+          flo = fm;
+        } else {
+          hi = mid;
+        }
+      }
+      x[found++] = 0.5 * (lo + hi);
+      a = b;
+      hermite_orthonormal(n, a, &fa, &ss);
+      if (fb == 0.0) a += 1e-12;
+      continue;
+    }
+    a = b;
+    fa = fb;
+  }
+  if (found != n) return FSM_ERR_UNSUPPORTED;
+  for (int k = 0; k < n; ++k) {
+    // symmetric rule: the two halves agree to rounding; average them
+    const double xs = 0.5 * (x[k] - x[n - 1 - k]);
+    double pn;
+    hermite_orthonormal(n, xs, &pn, &ss);
+    x[k] = xs;
+    w[k] = 1.0 / ss;
+  }
+  return FSM_OK;
+}
+
 void set_variant(Event2D &ev, int v) {
   ev.variant = v;
   ev.tiles_x = (ev.nx + fsm_stream2d_tile_w(v) - 1) / fsm_stream2d_tile_w(v);
@@ -204,6 +269,7 @@ static void *g_scratch_ptr[FSM_NOALLOC_MAX_DEVICES];
 struct fsm_ctx {
   int device = 0;
   int path = FSM_PATH_FP32;
+  int rapidity_rule = FSM_RAPIDITY_TAN;  // 3-D events: fsm_set_rapidity_rule
   int tile_policy = FSM_TILES_AUTO;
   int autotune = 1;
   int timing = 0;
@@ -773,13 +839,27 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     cs[2 * j] = cosf(phip);
     cs[2 * j + 1] = sinf(phip);
   }
+  const bool gh = ctx->rapidity_rule == FSM_RAPIDITY_GAUSS_HERMITE;
+  if (gh && p.BARYON) return FSM_ERR_UNSUPPORTED;
+  std::vector<double> ghx(nrap), ghw(nrap);
+  if (gh) {
+    const int st = gauss_hermite_rule(nrap, ghx.data(), ghw.data());
+    if (st) return st;
+  }
   auto wk = [&](int k) { return (float)(-.9975 + (double)(float)k * (1.995 / (double)(float)(nrap - 1))); };
-  auto rap_of = [&](float eta, int k) { return (float)((double)eta + tan((M_PI / 2.0) * (double)wk(k))); };
-  // rapidity factor of convertInitialDensity (src/FreeStream.cpp:342-345)
+  // rapidity node k of a cell at eta: the reference's tan grid
+  // (src/FreeStream.cpp:174-175), or eta + sqrt(2) SIGMA x_k
+  auto rap_of = [&](float eta, int k) {
+    if (gh) return (float)((double)eta + M_SQRT2 * (double)SIGMA * ghx[k]);
+    return (float)((double)eta + tan((M_PI / 2.0) * (double)wk(k)));
+  };
+  // rapidity factor of convertInitialDensity (src/FreeStream.cpp:342-345);
+  // under the Gauss-Hermite rule the Gaussian is in the weights
   auto rfac = [&](int ie, int k) {
     float eta = (float)ie * DETA + etamin;
     float rap = rap_of(eta, k);
     float d = eta - rap;
+    if (gh) return (float)((double)(coshf(d) * coshf(d)));
     return (float)((double)(coshf(d) * coshf(d)) *
                    exp((-1.0) * (double)d * (double)d / (2.0 * (double)SIGMA * (double)SIGMA)));
   };
@@ -805,8 +885,9 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
         S.D = D;
         S.rf0 = rfac(S.eta0, k);
         S.rf1 = rfac(S.eta0 + 1, k);
-        S.jac = (float)((M_PI / 2.0) / cos((M_PI / 2.0) * (double)w) / cos((M_PI / 2.0) * (double)w) *
-                        (1.995 / (double)(float)(nrap - 1)));
+        S.jac = gh ? (float)(M_SQRT2 * (double)SIGMA * ghw[k])
+                   : (float)((M_PI / 2.0) / cos((M_PI / 2.0) * (double)w) / cos((M_PI / 2.0) * (double)w) *
+                             (1.995 / (double)(float)(nrap - 1)));
         const float ch = coshf(rap - eta), th = tanhf(rap - eta);
         S.inv_ch = 1.0f / ch;
         S.inv_ch2 = 1.0f / (ch * ch);
@@ -958,10 +1039,16 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
 // Plans are cached per parameter set (most recently used first).  A batch
 // keeps shared references to the plans of its events, so eviction from the
 // cache never frees tables a recorded batch still uses.
+// A cached plan serves p on ctx: same parameters, and for 3-D events the
+// context's rapidity rule.
+bool plan_matches(const Plan &pl, const fsm_params &p, const fsm_ctx *ctx) {
+  return same_shape(pl.key, p) && (!pl.is3d || pl.rapidity_rule == ctx->rapidity_rule);
+}
+
 int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *out) {
   std::lock_guard<std::mutex> lock(ctx->plan_mu);
   for (size_t i = 0; i < ctx->plans.size(); ++i) {
-    if (same_shape(ctx->plans[i]->key, p)) {
+    if (plan_matches(*ctx->plans[i], p, ctx)) {
       std::shared_ptr<Plan> hit = ctx->plans[i];
       ctx->plans.erase(ctx->plans.begin() + i);
       ctx->plans.insert(ctx->plans.begin(), hit);
@@ -974,6 +1061,7 @@ int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *ou
   std::shared_ptr<Plan> pl(new Plan());
   pl->key = p;
   pl->is3d = p.DIM_ETA > 1;
+  pl->rapidity_rule = pl->is3d ? ctx->rapidity_rule : FSM_RAPIDITY_TAN;
   st = pl->is3d ? build_plan3d(ctx, p, *pl) : build_plan2d(ctx, p, *pl);
   if (st) return st;
   pl->valid = true;
@@ -1252,6 +1340,15 @@ void fsm_destroy(fsm_ctx *ctx) {
   if (ctx->last_call) (void)hipEventDestroy(ctx->last_call);
   delete ctx;
 }
+
+int fsm_set_rapidity_rule(fsm_ctx *ctx, int rule) {
+  if (!ctx || (rule != FSM_RAPIDITY_TAN && rule != FSM_RAPIDITY_GAUSS_HERMITE)) return FSM_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lock(ctx->call_mu);
+  ctx->rapidity_rule = rule;
+  return FSM_OK;
+}
+
+int fsm_gauss_hermite(int n, double *x, double *w) { return gauss_hermite_rule(n, x, w); }
 
 int fsm_set_path(fsm_ctx *ctx, int path) {
   if (!ctx) return FSM_ERR_ARG;
@@ -2370,7 +2467,7 @@ namespace {
 std::shared_ptr<Plan> find_plan(fsm_ctx *ctx, const fsm_params &p) {
   std::lock_guard<std::mutex> lock(ctx->plan_mu);
   for (const auto &pl : ctx->plans)
-    if (same_shape(pl->key, p)) return pl;
+    if (plan_matches(*pl, p, ctx)) return pl;
   return nullptr;
 }
 }  // namespace

@@ -49,6 +49,9 @@ FSM_ERR_NO_PLAN = -11
 
 PATH_FP32 = 0
 PATH_MIXED = 1
+# 3-D rapidity quadrature (fsm_set_rapidity_rule)
+RAPIDITY_TAN = 0
+RAPIDITY_GAUSS_HERMITE = 1
 
 IC_WHICH_ENERGY = 0
 IC_WHICH_BARYON = 1
@@ -169,6 +172,9 @@ def _lib():
         lib.fsm_destroy.argtypes = [vp]
         lib.fsm_destroy.restype = None
         lib.fsm_set_path.argtypes = [vp, ctypes.c_int]
+        lib.fsm_set_rapidity_rule.argtypes = [vp, ctypes.c_int]
+        lib.fsm_gauss_hermite.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
         lib.fsm_plan_describe.argtypes = [vp, P, ctypes.POINTER(PlanInfo)]
         lib.fsm_set_tile_policy.argtypes = [vp, ctypes.c_int]
         lib.fsm_set_autotune.argtypes = [vp, ctypes.c_int]
@@ -231,6 +237,14 @@ def _lib():
 def lib():
     """The loaded ctypes library (raises ImportError when not built)."""
     return _lib()
+
+
+def gauss_hermite(n: int):
+    """The library's n-point Gauss-Hermite rule (weight exp(-x^2)): (x, w)."""
+    x = (ctypes.c_double * n)()
+    w = (ctypes.c_double * n)()
+    _check(_lib().fsm_gauss_hermite(n, x, w), "fsm_gauss_hermite")
+    return np.array(x), np.array(w)
 
 
 def version() -> str:
@@ -353,6 +367,10 @@ class Context:
 
     def set_path(self, path: int):
         _check(_lib().fsm_set_path(self._ctx, path), "fsm_set_path")
+
+    def set_rapidity_rule(self, rule: int):
+        """RAPIDITY_TAN (the reference's grid) or RAPIDITY_GAUSS_HERMITE (3-D events)."""
+        _check(_lib().fsm_set_rapidity_rule(self._ctx, rule), "fsm_set_rapidity_rule")
 
     # -- host-buffer step ------------------------------------------------
     def run(self, p: Params, e: np.ndarray):

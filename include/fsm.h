@@ -135,6 +135,27 @@ enum {
 };
 int fsm_set_path(fsm_ctx *ctx, int path);
 
+/* Rapidity quadrature of 3-D events (no counterpart in the reference).
+ * FSM_RAPIDITY_TAN is the reference's rule: nodes y = eta + tan(pi/2 w) on a
+ * uniform w grid, weights the Jacobian dy/dw (src/FreeStream.cpp:174-175,
+ * :340-345; src/LandauMatch.cpp:93-95).  FSM_RAPIDITY_GAUSS_HERMITE replaces
+ * them by the DIM_RAP-point Gauss-Hermite rule for the Gaussian rapidity
+ * profile of convertInitialDensity: nodes y = eta + sqrt(2) SIGMA x_k and
+ * weights sqrt(2) SIGMA W_k (x_k, W_k for the weight exp(-x^2), as in the
+ * reference's quadrature_tables/gaussHermite*.dat), the profile's
+ * exp(-(eta - y)^2 / (2 SIGMA^2)) moving from the density into the weights
+ * (the density keeps norm * cosh^2(eta - y)).  Applies to the energy pass of
+ * events with DIM_ETA > 1; an event with BARYON = 1 is refused
+ * (FSM_ERR_UNSUPPORTED: the reference's charge pass shares the energy pass's
+ * streaming geometry).  Context-wide; plans are kept per rule. */
+enum { FSM_RAPIDITY_TAN = 0, FSM_RAPIDITY_GAUSS_HERMITE = 1 };
+int fsm_set_rapidity_rule(fsm_ctx *ctx, int rule);
+
+/* Nodes x[0..n) (ascending) and weights w[0..n) of the n-point Gauss-Hermite
+ * rule for the weight exp(-x^2) (no counterpart in the reference, which only
+ * ships the 21-, 31- and 41-point tables as data files).  n in [1, 200]. */
+int fsm_gauss_hermite(int n, double *x, double *w);
+
 /* What the planner chose for an event shape (no counterpart in the
  * reference; for tuning reports).  Plans are cached per parameter set. */
 typedef struct fsm_plan_info {

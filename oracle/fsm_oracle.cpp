@@ -463,6 +463,24 @@ enum Mutation {
 int g_mutation = MUT_NONE;
 inline bool mut(int m) { return g_mutation == m; }
 
+// Rapidity rule of the 3-D path.  0: the reference's tan grid (the only rule
+// the reference has).  1: the Gauss-Hermite rule of the library's
+// FSM_RAPIDITY_GAUSS_HERMITE extension, nodes eta + sqrt(2) SIGMA x_k and
+// weights sqrt(2) SIGMA W_k with the density keeping norm * cosh^2(eta - y);
+// x_k, W_k are handed in by oracle_set_rapidity_rule (the tests take them
+// from numpy, independently of the library's own rule).
+int g_rap_rule = 0;
+std::vector<double> g_gh_x, g_gh_w;
+
+// Rapidity node irap of a cell at eta (src/FreeStream.cpp:174-175 and
+// :337-340, src/LandauMatch.cpp:37-38 under the tan grid).
+template <typename R>
+R rap_node(R eta, int irap, int DIM_RAP, double sigma) {
+  if (g_rap_rule == 1) return (R)((double)eta + sqrt(2.0) * sigma * g_gh_x[irap]);
+  R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
+  return (R)((double)eta + tan((M_PI / 2.0) * (double)w));
+}
+
 // EquationOfState.cpp:5-80, one cell.
 template <typename R>
 R pressure_of(R e, int eos) {
@@ -529,11 +547,11 @@ struct Pipeline {
         int ieta = is - (DIM_Y * DIM_ETA * ix) - (DIM_ETA * iy);
         R eta = (R)ieta * DETA + etamin;
         for (int irap = 0; irap < DIM_RAP; ++irap) {
-          R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
-          R rap = (R)((double)eta + tan((M_PI / 2.0) * (double)w));
+          R rap = rap_node<R>(eta, irap, DIM_RAP, (double)SIGMA);
           R d = eta - rap;
-          R rap_factor = (R)((double)(std::cosh(d) * (mut(MUT_RAP_FACTOR_COSH) ? (R)1 : std::cosh(d))) *
-                             exp((-1.0) * (double)d * (double)d / (2.0 * (double)SIGMA * (double)SIGMA)));
+          const double gauss =
+              g_rap_rule == 1 ? 1.0 : exp((-1.0) * (double)d * (double)d / (2.0 * (double)SIGMA * (double)SIGMA));
+          R rap_factor = (R)((double)(std::cosh(d) * (mut(MUT_RAP_FACTOR_COSH) ? (R)1 : std::cosh(d))) * gauss);
           dens[(size_t)is * DIM_RAP + irap] = e0[is] * norm_factor * rap_factor;
         }
       }
@@ -603,8 +621,7 @@ struct Pipeline {
       R y = (R)iy * DY + ymin;
       R eta = (R)ieta * DETA + etamin;
       for (int irap = 0; irap < DIM_RAP; ++irap) {
-        R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
-        R rap = (R)((double)eta + tan((M_PI / 2.0) * (double)w));
+        R rap = rap_node<R>(eta, irap, DIM_RAP, P.SIGMA);
         for (int iphip = 0; iphip < DIM_PHIP; ++iphip) {
           R phip = (R)((double)(R)iphip * (2.0 * M_PI) / (double)(R)DIM_PHIP);
           R eta_new, x_new, y_new;
@@ -677,8 +694,7 @@ struct Pipeline {
         for (int ieta = 0; ieta < DIM_ETA; ++ieta) {
           R eta = (R)ieta * DETA + etamin;
           if (DIM_ETA == 1) eta = 0.0;
-          R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
-          R rap = (R)((double)eta + tan((M_PI / 2.0) * (double)w));
+          R rap = rap_node<R>(eta, irap, DIM_RAP, P.SIGMA);
           if (DIM_ETA == 1) rap = 0.0;
           R c = std::cos(phip), s = std::sin(phip);
           R ch = std::cosh(rap - eta), th = std::tanh(rap - eta);
@@ -712,9 +728,11 @@ struct Pipeline {
         R acc = T[(size_t)ivar * DIM + is];
         for (int irap = 0; irap < DIM_RAP; ++irap) {
           R w = (R)(-.9975 + (double)(R)irap * (1.995 / (double)(R)(DIM_RAP - 1)));
-          R jacobian = (R)((M_PI / 2.0) / cos((M_PI / 2.0) * (double)w) /
-                           (mut(MUT_JACOBIAN_ONE_COS) ? 1.0 : cos((M_PI / 2.0) * (double)w)) *
-                           (1.995 / (double)(R)(DIM_RAP - 1)));
+          R jacobian = g_rap_rule == 1
+                           ? (R)(sqrt(2.0) * P.SIGMA * g_gh_w[irap])
+                           : (R)((M_PI / 2.0) / cos((M_PI / 2.0) * (double)w) /
+                                 (mut(MUT_JACOBIAN_ONE_COS) ? 1.0 : cos((M_PI / 2.0) * (double)w)) *
+                                 (1.995 / (double)(R)(DIM_RAP - 1)));
           for (int iphip = 0; iphip < DIM_PHIP; ++iphip) {
             R s = shifted[((size_t)is * DIM_RAP + irap) * DIM_PHIP + iphip];
             R h = H[(((size_t)ivar * DIM_RAP + irap) * DIM_PHIP + iphip) * DIM_ETA + ieta];
@@ -952,6 +970,20 @@ void oracle_pressure(int eos, int precision, long n, const double *e, double *p)
 void oracle_viscous_float(const fsm_params *P, const float *T, const float *e, const float *u, const float *p,
                           float *Pi, float *pi) {
   viscous_stages<float>(P->DIM, P->TAU, T, e, u, p, Pi, pi);
+}
+
+// Rapidity rule of the 3-D path: rule 0 (the reference's tan grid), or 1
+// with the n Gauss-Hermite nodes x and weights w for the weight exp(-x^2).
+int oracle_set_rapidity_rule(int rule, int n, const double *x, const double *w) {
+  if (rule == 0) {
+    g_rap_rule = 0;
+    return 0;
+  }
+  if (rule != 1 || n < 1 || !x || !w) return -1;
+  g_gh_x.assign(x, x + n);
+  g_gh_w.assign(w, w + n);
+  g_rap_rule = 1;
+  return 0;
 }
 
 // Plants mistake `m` (enum Mutation) for the tests of the pins; 0 restores

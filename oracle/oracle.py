@@ -78,12 +78,44 @@ def params_finalize(p, e=None):
     return q
 
 
-def run(p, e, precision: str = "float", nB=None, want_T: bool = True, want_extra: bool = True):
+class gauss_hermite:
+    """Context manager switching the oracle's 3-D rapidity rule to the
+    n-point Gauss-Hermite rule of the library's FSM_RAPIDITY_GAUSS_HERMITE
+    extension (nodes eta + sqrt(2) SIGMA x_k, weights sqrt(2) SIGMA W_k).  The
+    nodes and weights come from numpy.polynomial.hermite.hermgauss, not from
+    the library.  The reference's tan grid is restored on exit."""
+
+    def __init__(self, n: int):
+        self.x, self.w = np.polynomial.hermite.hermgauss(int(n))
+
+    def __enter__(self):
+        lib = _lib()
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.oracle_set_rapidity_rule.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        x = np.ascontiguousarray(self.x, dtype=np.float64)
+        w = np.ascontiguousarray(self.w, dtype=np.float64)
+        if lib.oracle_set_rapidity_rule(1, x.size, _d(x), _d(w)) != 0:
+            raise ValueError("oracle_set_rapidity_rule failed")
+        return self
+
+    def __exit__(self, *exc):
+        _lib().oracle_set_rapidity_rule(0, 0, None, None)
+        return False
+
+
+def run(p, e, precision: str = "float", nB=None, want_T: bool = True, want_extra: bool = True,
+        rapidity: str = "tan"):
     """Whole step.  precision 'float' = reference float path, 'double' = FP64 oracle.
 
     p must be finalized.  Returns dict with 'out' (17 x DIM, output_to_vectors
     order), 'T' (10 x DIM, fm^-4), 'e', 'u' (fm^-4 / dimensionless), optional
-    'nB', 'V' (baryon), and 'stage_seconds'."""
+    'nB', 'V' (baryon), and 'stage_seconds'.  rapidity 'gauss_hermite' runs a
+    3-D event under the Gauss-Hermite rule (see gauss_hermite)."""
+    if rapidity == "gauss_hermite":
+        with gauss_hermite(p.DIM_RAP):
+            return run(p, e, precision, nB, want_T, want_extra)
+    if rapidity != "tan":
+        raise ValueError("rapidity rule %r" % rapidity)
     e = np.ascontiguousarray(e, dtype=np.float32).ravel()
     D = p.DIM
     if e.size != D:
