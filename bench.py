@@ -479,6 +479,36 @@ def main():
                       "quadrature measured slower and is not built); stream_tflops counts the reference's 36 FLOP "
                       "per sample the masks keep (kept of nominal DIM*DIM_RAP*DIM_PHIP; evaluated = what the "
                       "kernel computes); landau_hbm = 40 B in + 136 B out per cell over the Landau stage time")
+        # the same event under the 21-point Gauss-Hermite rapidity rule (the configuration's rule;
+        # the reference's own rule, above, is the tan grid), mixed path
+        ctx.set_path(fsm_amd.PATH_MIXED)
+        ctx.set_rapidity_rule(fsm_amd.RAPIDITY_GAUSS_HERMITE)
+        info_gh = ctx.plan_info(q2)
+        ctx.run_device(q2, d2.data_ptr(), o2.data_ptr(), stream)
+        sync()
+        lat_gh = []
+        for _ in range(args.c2_steps):
+            a = time.perf_counter()
+            ctx.run_device(q2, d2.data_ptr(), o2.data_ptr(), stream)
+            stream.synchronize()
+            lat_gh.append((time.perf_counter() - a) * 1e3)
+        ctx.enable_timing(True)
+        ctx.kernel_time_totals()
+        timed(lambda: ctx.run_device(q2, d2.data_ptr(), o2.data_ptr(), stream), args.c2_steps, sync)
+        kt_gh, n_gh = ctx.kernel_time_totals()
+        ctx.enable_timing(False)
+        ctx.set_rapidity_rule(fsm_amd.RAPIDITY_TAN)
+        lat_gh = np.array(lat_gh)
+        c2["gauss_hermite_mixed"] = {
+            "events_per_s": round(1e3 / float(lat_gh.mean()), 2), "ms_per_event": round(float(lat_gh.mean()), 4),
+            "latency_ms": {"p50": round(float(np.percentile(lat_gh, 50)), 4),
+                           "p99": round(float(np.percentile(lat_gh, 99)), 4)},
+            "stage_ms": {"src": round(kt_gh[0] / n_gh, 4), "stream": round(kt_gh[1] / n_gh, 4),
+                         "landau": round(kt_gh[2] / n_gh, 4)},
+            "samples": {"kept": int(info_gh["samples_kept"]), "evaluated": int(info_gh["samples_evaluated"])},
+            "note": "rapidity nodes eta + sqrt(2) SIGMA x_k, weights sqrt(2) SIGMA W_k of the 21-point "
+                    "Gauss-Hermite rule (fsm_set_rapidity_rule); parity against the oracle under the same rule "
+                    "in tests/test_gpu_gauss_hermite.py"}
         del d2, o2
 
     # ---- C3: a batch of 512 mixed 2-D events --------------------------------------

@@ -1079,12 +1079,12 @@ int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *ou
 // Batched device time per unit of work on MI355X, in picoseconds, from the
 // measured stage times of the 261^2 lumpy event (64-event batches: stream
 // 14.4 / 8.5 us for 13.6 M samples, spline 1.5 us and Landau 3.0 us for 68 k
-// cells) and of the 131^2 x 101 event (stream 0.76 ms for 6.8e8 evaluated
+// cells) and of the 131^2 x 101 event (stream 0.47 ms for 6.8e8 evaluated
 // samples, Landau 0.14 ms for 1.73 M cells): 2-D stream per (cell, phi)
 // sample on each path, 3-D stream per evaluated sample, spline or source
 // planes and Landau per cell.
 const double kCostStream2dPs[2] = {1.06, 0.62};  // FP32, mixed
-const double kCostStream3dPs = 1.12;
+const double kCostStream3dPs = 0.70;
 const double kCostPlanes2dPs = 22.0, kCostPlanes3dPs = 17.0;
 const double kCostLandau2dPs = 36.0, kCostLandau3dPs = 81.0;
 
