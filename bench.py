@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--serve-outstanding", type=int, default=256,
                     help="requests kept outstanding in the closed-loop (capacity) serving run")
     ap.add_argument("--serve-pool", type=int, default=40, help="distinct requests in the serving pool")
+    ap.add_argument("--serve-rapidity", choices=("gauss_hermite", "tan"), default="gauss_hermite",
+                    help="rapidity rule of the pool's 3-D requests (the 3-D configuration names a 21-point "
+                         "Gauss-Hermite rule; tan = the reference's own grid)")
     ap.add_argument("--c3-events", type=int, default=512)
     ap.add_argument("--c3-steps", type=int, default=6)
     ap.add_argument("--no-c3", action="store_true")
@@ -521,6 +524,8 @@ def main():
     serve = None
     if not args.no_serve:
         ctx.set_path(fsm_amd.PATH_MIXED)
+        ctx.set_rapidity_rule(fsm_amd.RAPIDITY_GAUSS_HERMITE if args.serve_rapidity == "gauss_hermite"
+                              else fsm_amd.RAPIDITY_TAN)
         pool = W.serving_mix(args.serve_pool, seed=11)
         n3 = sum(1 for p, _, _ in pool if p.DIM_ETA > 1)
         nb = sum(1 for _, _, b in pool if b is not None)
@@ -537,11 +542,14 @@ def main():
         plain = plain_pool_run(fsm_amd, ctx, pool, dev_pool, stream)
         ctx.set_path(fsm_amd.PATH_MIXED)
         plain_mixed = plain_pool_run(fsm_amd, ctx, pool, dev_pool, stream)
+        ctx.set_rapidity_rule(fsm_amd.RAPIDITY_TAN)
         del dev_pool
         serve = {"workload": "Poisson arrivals for %.1f s per rate from a pool of %d requests: %d 3-D (131^2 x "
-                             "DIM_ETA 41-101, DIM_RAP 21), the rest C3-style 2-D; %d carry n_B (BARYON=1, "
-                             "SIGMA_B=0.8); profiles resident in HBM (submit_device), one rate also from host "
-                             "memory; results kept in HBM; mixed path" % (args.serve_seconds, len(pool), n3, nb),
+                             "DIM_ETA 41-101, DIM_RAP 21, %s rapidity rule), the rest C3-style 2-D; %d carry n_B "
+                             "(BARYON=1, SIGMA_B=0.8); profiles resident in HBM (submit_device), one rate also from "
+                             "host memory; results kept in HBM; mixed path"
+                             % (args.serve_seconds, len(pool), n3, args.serve_rapidity.replace("_", "-"), nb),
+                 "rapidity_rule_3d": args.serve_rapidity,
                  "sweep": sweep, "capacity_closed_loop": capacity, "host_inputs": host_point,
                  "plain_fp32_same_pool": plain, "plain_mixed_same_pool": plain_mixed,
                  "serving_over_plain_fp32": round(capacity["events_per_s"] / plain["events_per_s"], 3),

@@ -840,7 +840,6 @@ int build_plan3d(fsm_ctx *ctx, const fsm_params &p, Plan &pl) {
     cs[2 * j + 1] = sinf(phip);
   }
   const bool gh = ctx->rapidity_rule == FSM_RAPIDITY_GAUSS_HERMITE;
-  if (gh && p.BARYON) return FSM_ERR_UNSUPPORTED;
   std::vector<double> ghx(nrap), ghw(nrap);
   if (gh) {
     const int st = gauss_hermite_rule(nrap, ghx.data(), ghw.data());

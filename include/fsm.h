@@ -144,10 +144,12 @@ int fsm_set_path(fsm_ctx *ctx, int path);
  * weights sqrt(2) SIGMA W_k (x_k, W_k for the weight exp(-x^2), as in the
  * reference's quadrature_tables/gaussHermite*.dat), the profile's
  * exp(-(eta - y)^2 / (2 SIGMA^2)) moving from the density into the weights
- * (the density keeps norm * cosh^2(eta - y)).  Applies to the energy pass of
- * events with DIM_ETA > 1; an event with BARYON = 1 is refused
- * (FSM_ERR_UNSUPPORTED: the reference's charge pass shares the energy pass's
- * streaming geometry).  Context-wide; plans are kept per rule. */
+ * (the density keeps norm * cosh^2(eta - y)).  Applies to events with
+ * DIM_ETA > 1.  The charge pass of BARYON = 1 streams with the same nodes (in
+ * the reference it shares the energy pass's geometry, src/FreestreamMilne.cpp:
+ * 386) and keeps its density on the uniform DRAP grid and its weight DRAP
+ * (src/FreeStream.cpp:353-387, src/LandauMatch.cpp:110-143), as the reference
+ * does under the tan grid.  Context-wide; plans are kept per rule. */
 enum { FSM_RAPIDITY_TAN = 0, FSM_RAPIDITY_GAUSS_HERMITE = 1 };
 int fsm_set_rapidity_rule(fsm_ctx *ctx, int rule);
 

@@ -110,18 +110,36 @@ def test_rules_keep_their_own_plans():
         ctx.close()
 
 
-def test_gauss_hermite_refuses_baryon_events(gh_ctx):
+@pytest.mark.parametrize("path", ["fp32", "mixed"])
+def test_gauss_hermite_baryon_parity(gh_ctx, path):
+    """The charge pass streams with the Gauss-Hermite nodes too (the
+    reference's charge pass shares the energy pass's geometry) and keeps the
+    uniform DRAP grid for its density and weight."""
     import torch
 
-    p, e = W.small_3d(nrap=9)
+    gh_ctx.set_path(fsm_amd.PATH_MIXED if path == "mixed" else fsm_amd.PATH_FP32)
+    p, e = W.small_3d(nrap=11, seed=23)
     q = fsm_amd.params_finalize(W.with_baryon(p), e)
     nb = W.baryon_density(e)
     d_e = torch.from_numpy(e).cuda()
     d_nb = torch.from_numpy(nb).cuda()
-    d_out = torch.empty(22 * q.DIM, dtype=torch.float64, device="cuda")
-    with pytest.raises(fsm_amd.FsmError) as exc:
-        gh_ctx.run_baryon_device(q, d_e.data_ptr(), d_nb.data_ptr(), d_out.data_ptr(), torch.cuda.current_stream())
-    assert exc.value.status == fsm_amd.FSM_ERR_UNSUPPORTED
+    D = q.DIM
+    d_out = torch.empty(fsm_amd.NUM_OUTPUTS_BARYON * D, dtype=torch.float64, device="cuda")
+    gh_ctx.run_baryon_device(q, d_e.data_ptr(), d_nb.data_ptr(), d_out.data_ptr(), torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy().reshape(-1, D)
+    ref = O.run(q, e, "float", nB=nb, rapidity="gauss_hermite")
+    T00max = np.abs(ref["T"][0]).max()
+    cond = well_conditioned(ref["T"], q.TAU, path)
+    errs = output_errs(out[:17], ref["out"], T00max, cond)
+    nbmax = np.abs(ref["nB"]).max()
+    keep = cond & (ref["out"][0] > 1e-3 * ref["out"][0].max())
+    dn = np.abs(out[17] - ref["nB"])[keep].max() / nbmax
+    dv = max(np.abs(out[18 + k] - ref["V"][k])[keep].max() for k in range(4)) / nbmax
+    print("baryon GH", path, errs, "nB %.3g V %.3g" % (dn, dv), flush=True)
+    for k in ("e", "u", "pi", "Pi"):
+        assert errs[k] <= TOL, (k, errs[k])
+    assert dn <= TOL and dv <= TOL
 
 
 def test_gauss_hermite_batch_matches_single_events(gh_ctx):
