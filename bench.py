@@ -87,6 +87,10 @@ def parse():
     ap.add_argument("--profile-pass", action="store_true",
                     help="only launch what a counter profile needs: a 256 MiB fill and a 256 MiB elementwise "
                          "copy (the byte-counter calibration), then one eager C1 batch on each path")
+    ap.add_argument("--profile-3d", action="store_true",
+                    help="only launch what a 3-D counter profile needs: the byte-counter calibration kernels, then "
+                         "the 131^2 x 101 event once on the FP32 path under the reference's rapidity grid and once "
+                         "under the 21-point Gauss-Hermite rule")
     return ap.parse_args()
 
 
@@ -121,6 +125,34 @@ def profile_pass(args):
         bt.close()
     ctx.close()
     print(json.dumps({"profile_pass": True, "copy_bytes": 4 * n, "batch": B}), flush=True)
+
+
+def profile_3d():
+    """Short run for rocprofv3 --pmc on the 3-D kernels: the calibration
+    kernels of profile_pass, then the 131^2 x 101 event (one-event launches,
+    FP32 path) under each rapidity rule."""
+    import torch
+    import fsm_amd
+    import fsm_amd.workloads as W
+
+    n = 64 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    ctx = fsm_amd.Context(0)
+    ctx.set_path(fsm_amd.PATH_FP32)
+    p2, e2 = W.config("C2")
+    q2 = fsm_amd.params_finalize(p2, e2)
+    d_e = torch.from_numpy(e2).cuda()
+    d_out = torch.empty(17 * q2.DIM, dtype=torch.float64, device="cuda")
+    for rule in (fsm_amd.RAPIDITY_TAN, fsm_amd.RAPIDITY_GAUSS_HERMITE):
+        ctx.set_rapidity_rule(rule)
+        ctx.run_device(q2, d_e.data_ptr(), d_out.data_ptr(), torch.cuda.current_stream())
+        torch.cuda.synchronize()
+    ctx.close()
+    print(json.dumps({"profile_3d": True, "copy_bytes": 4 * n}), flush=True)
 
 
 def timed(fn, steps, sync):
@@ -302,6 +334,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(events shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.profile_3d:
+        profile_3d()
+        return
     if args.profile_pass:
         profile_pass(args)
         return
