@@ -75,8 +75,8 @@ def parse():
                     help="requests kept outstanding in the closed-loop (capacity) serving run")
     ap.add_argument("--serve-pool", type=int, default=40, help="distinct requests in the serving pool")
     ap.add_argument("--serve-rapidity", choices=("gauss_hermite", "tan"), default="gauss_hermite",
-                    help="rapidity rule of the pool's 3-D requests (the 3-D configuration names a 21-point "
-                         "Gauss-Hermite rule; tan = the reference's own grid)")
+                    help="rapidity rule of the pool's 3-D requests: the 21-point Gauss-Hermite rule, or tan = "
+                         "the reference's own grid")
     ap.add_argument("--c3-events", type=int, default=512)
     ap.add_argument("--c3-steps", type=int, default=6)
     ap.add_argument("--no-c3", action="store_true")
@@ -517,8 +517,8 @@ def main():
                       "quadrature measured slower and is not built); stream_tflops counts the reference's 36 FLOP "
                       "per sample the masks keep (kept of nominal DIM*DIM_RAP*DIM_PHIP; evaluated = what the "
                       "kernel computes); landau_hbm = 40 B in + 136 B out per cell over the Landau stage time")
-        # the same event under the 21-point Gauss-Hermite rapidity rule (the configuration's rule;
-        # the reference's own rule, above, is the tan grid), mixed path
+        # the same event under the 21-point Gauss-Hermite rapidity rule (the reference's own rule, above, is
+        # the tan grid), mixed path
         ctx.set_path(fsm_amd.PATH_MIXED)
         ctx.set_rapidity_rule(fsm_amd.RAPIDITY_GAUSS_HERMITE)
         info_gh = ctx.plan_info(q2)

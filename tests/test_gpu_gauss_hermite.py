@@ -2,8 +2,8 @@
 the oracle run under the same rule (its nodes from numpy, see
 tests/test_gauss_hermite.py), with the tolerances of tests/parity.py.  The
 reference's own rule is the tan grid (src/FreeStream.cpp:174-175); the
-benchmark's 3-D configuration names a 21-point Gauss-Hermite rule with
-SIGMA = 1, which is this extension."""
+Gauss-Hermite rule integrates the Gaussian rapidity profile of
+convertInitialDensity exactly for polynomial factors."""
 import numpy as np
 import pytest
 
