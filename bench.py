@@ -183,7 +183,7 @@ def serve_run(fsm_amd, W, ctx, seconds, rate, pool, dev_pool=None, seed=5, close
     instead of Poisson arrivals, keep that many requests outstanding (a new one
     is submitted as each completes): the server's capacity."""
     rng = np.random.default_rng(seed)
-    srv = fsm_amd.Server(ctx)  # defaults: 5 launch slots (three for 2-D work only), 300 us of work per launch
+    srv = fsm_amd.Server(ctx)  # defaults: 5 launch slots (three for 2-D work only), 1 ms of work per launch
 
     def submit(k):
         p, e, nb = pool[k]

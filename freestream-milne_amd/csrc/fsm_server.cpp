@@ -263,7 +263,12 @@ int fsm_server_create(fsm_ctx *ctx, const fsm_server_config *cfg, fsm_server **o
   if (s->cfg.max_batch_events <= 0) s->cfg.max_batch_events = 256;
   if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 5;
   if (s->cfg.max_requests <= 0) s->cfg.max_requests = 4096;
-  if (!(s->cfg.max_batch_us > 0.0f)) s->cfg.max_batch_us = 300.0f;
+  // 1 ms of estimated work per launch: a launch costs tens of microseconds of
+  // host time (descriptors, uploads, one kernel launch per stage), so small
+  // caps bound the request rate; at 3000 requests/s of the mixed 2-D/3-D pool
+  // the p99 latency is the same with 0.3 and 1.2 ms caps (about 1.1 ms) and
+  // the capacity 18% higher with the larger one
+  if (!(s->cfg.max_batch_us > 0.0f)) s->cfg.max_batch_us = 1000.0f;
   int st = FSM_OK;
   if (hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking) != hipSuccess) st = FSM_ERR_DEVICE;
   s->slots.resize(s->cfg.max_inflight);

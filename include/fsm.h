@@ -364,7 +364,7 @@ typedef struct fsm_server_config {
   int max_inflight;        /* launches in flight (0: 5); with two or more, all but the last two
                               (at least one) take only 2-D requests */
   int max_requests;        /* admitted + completed-unreleased requests (0: 4096) */
-  float max_batch_us;      /* estimated device time per launch, microseconds (0: 300); a
+  float max_batch_us;      /* estimated device time per launch, microseconds (0: 1000); a
                               launch always takes at least one request */
 } fsm_server_config;
 
