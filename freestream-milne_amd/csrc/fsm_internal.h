@@ -75,7 +75,15 @@ __device__ __forceinline__ uint2 fsm_gload(const uint2 *p) {
   return make_uint2(v.x, v.y);
 }
 __device__ __forceinline__ unsigned fsm_gload(const unsigned *p) { return *(const FSM_AS_GLOBAL unsigned *)p; }
+__device__ __forceinline__ float fsm_gload(const float *p) { return *(const FSM_AS_GLOBAL float *)p; }
 __device__ __forceinline__ void fsm_gstore(float *p, float v) { *(FSM_AS_GLOBAL float *)p = v; }
+__device__ __forceinline__ void fsm_gstore(uint4 *p, uint4 v) {
+  *(FSM_AS_GLOBAL fsm_v4u *)p = fsm_v4u{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void fsm_gstore(float4 *p, float4 v) {
+  *(FSM_AS_GLOBAL fsm_v4u *)p =
+      fsm_v4u{__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+}
 // Copies 16 bytes per lane from global memory straight into LDS
 // (global_load_lds_dwordx4): lane l of the wave writes lds_base[l], with
 // lds_base wave-uniform and 16-byte aligned.  Inactive lanes write nothing.

@@ -221,7 +221,9 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
       const int gx = x0 - B + wave + 4 * u;
-      z[u] = (yin && gx >= 0 && gx < nx) ? knot_z(ev, ev.e_gev[(long)gx * ny + gy]) : 0.0f;
+      z[u] = (yin && gx >= 0 && gx < nx)
+                 ? knot_z(ev, fsm_gload(ev.e_gev + (long)gx * ny + gy))
+                 : 0.0f;
     }
     // largest |z| over each flag tile's band region (staged rows [16 i, 16 i + 47])
     float zm[NF];
@@ -327,7 +329,7 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
       bool bad = false;
       if (live) {
         const float4 v = stg[row * SP2_OP + col];
-        ev.planes[(long)gx * ny + gy] = v;
+        fsm_gstore(ev.planes + (long)gx * ny + gy, v);
         if (ev.planes_h) {
           const int fi = row / FSMSP_TX;
           float zmx = 0.0f;
@@ -344,9 +346,9 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
           split_half(v.y * sc, h[1], l[1]);
           split_half(v.z * sc, h[2], l[2]);
           split_half(v.w * sc, h[3], l[3]);
-          ev.planes_h[(long)gx * ny + gy] =
-              make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
-                         l[0] | ((unsigned)l[1] << 16), l[2] | ((unsigned)l[3] << 16));
+          fsm_gstore(ev.planes_h + (long)gx * ny + gy,
+                     make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
+                                l[0] | ((unsigned)l[1] << 16), l[2] | ((unsigned)l[3] << 16)));
         }
       }
       if (ev.planes_h && __ballot(bad) != 0ull && lane == 0) atomicOr(&L.bad[row / FSMSP_TX], 1u);
@@ -413,8 +415,8 @@ __global__ __launch_bounds__(256) void k_knot_scale_batch(const Event2D *__restr
                                                           const int4 *__restrict__ items, int n_items) {
   __shared__ float red[4];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const int4 item = items[xcd_order(it, n_items)];
-    const Event2D ev = evs[item.x];
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const Event2D ev = fsm_sload(evs + item.x);
     if (ev.planes_h) knot_scale_part(ev, item.y, red);
     __syncthreads();
   }
@@ -432,8 +434,9 @@ __global__ __launch_bounds__(256) void k_spline2d_batch(const Event2D *__restric
                                                         const int4 *__restrict__ items, int n_items) {
   __shared__ SplineLds2 L;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[xcd_order(it, n_items)];
-    const Event2D ev = evs[item.x];
+    // descriptors through scalar loads (uniform; written before the launch)
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const Event2D ev = fsm_sload(evs + item.x);
     spline2d_tile2(ev, item.y, item.z, L);
   }
 }
@@ -480,8 +483,8 @@ __global__ __launch_bounds__(256) void k_src3d_batch(const Event3D *__restrict__
                                                      const int4 *__restrict__ items, int n_items) {
   __shared__ float tile[64][65];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[xcd_order(it, n_items)];
-    const Event3D ev = evs[item.x];
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const Event3D ev = fsm_sload(evs + item.x);
     src3d_tile(ev.e_gev, ev.src, ev.nx * ev.ny, ev.neta, ev.norm_factor, ev.charge | ev.raw_e, item.y, item.z,
                tile);
   }
@@ -1304,8 +1307,8 @@ __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restr
   extern __shared__ float lds_out[];
   __shared__ LandauShared S;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[xcd_order(it, n_items)];
-    const EventLandau ev = evs[item.x];
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const EventLandau ev = fsm_sload(evs + item.x);
     if (item.w) {
       landau3d_tile(ev, item.y, item.z, lds_out, S);
     } else {
@@ -1322,8 +1325,8 @@ __global__ __launch_bounds__(256) void k_landau2d_batch(const EventLandau *__res
                                                         const int4 *__restrict__ items, int n_items) {
   __shared__ LandauShared S;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    int4 item = items[xcd_order(it, n_items)];
-    const EventLandau ev = evs[item.x];
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const EventLandau ev = fsm_sload(evs + item.x);
     for (int base = 0; base < item.z; base += blockDim.x)
       landau_cell_out<3>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
   }

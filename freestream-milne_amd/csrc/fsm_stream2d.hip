@@ -657,8 +657,8 @@ __global__ __launch_bounds__(256) void k_stream2d_batch(const Event2D *__restric
                                                         const int4 *__restrict__ items, int n_items) {
   extern __shared__ float4 lds4[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const int4 item = items[xcd_order(it, n_items)];
-    const Event2D ev = evs[item.x];
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const Event2D ev = fsm_sload(evs + item.x);
     switch (ev.variant) {
       case 1: stream2d_tile_v2<4, 4>(ev, item.y, item.z, lds4); break;
       case 2: stream2d_tile_v2<2, 4>(ev, item.y, item.z, lds4); break;
@@ -676,8 +676,8 @@ __global__ __launch_bounds__(256, 3) void k_stream2d_mfma_batch(const Event2D *_
                                                              const int4 *__restrict__ items, int n_items) {
   extern __shared__ float4 lds4[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const int4 item = items[xcd_order(it, n_items)];
-    const Event2D ev = evs[item.x];
+    const int4 item = fsm_sload(items + xcd_order(it, n_items));
+    const Event2D ev = fsm_sload(evs + item.x);
     switch (ev.variant) {
       case 1: stream2d_mfma_tile<1>(ev, item.y, item.z, lds4); break;
       case 2: stream2d_mfma_tile<2>(ev, item.y, item.z, lds4); break;
