@@ -638,7 +638,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16 hi+lo MFMA, fp32 accumulate; fp32/fp64 Landau",
+        "dtype": "fp16 hi+lo MFMA, fp32 accumulate; fp64 Landau frame",
         "data": "synthetic",
         "config": {"workload": "2-D lumpy event 261x261, DX=0.06, 200 hotspots, DIM_PHIP=200, DTAU=1.0; "
                                "%d events per step, device-resident, batched launch replayed from a HIP graph, "

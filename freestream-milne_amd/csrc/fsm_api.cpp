@@ -1191,7 +1191,7 @@ int stream_charge(fsm_ctx *ctx, Plan &pl, const fsm_params &p, const float *d_nb
   return FSM_OK;
 }
 
-EventLandau landau_event(const fsm_params &p, const float *T, int eta_major, int mixed) {
+EventLandau landau_event(const fsm_params &p, const float *T, int eta_major) {
   EventLandau L;
   memset(&L, 0, sizeof(L));
   L.dim = p.DIM;
@@ -1201,7 +1201,6 @@ EventLandau landau_event(const fsm_params &p, const float *T, int eta_major, int
   L.eos_type = p.EOS_TYPE;
   L.viscous = p.VISCOUS_MATCHING;
   L.fill_outputs = (p.OUTPUTFORMAT == 2 || p.OUTPUTFORMAT == 3) ? 1 : 0;
-  L.mixed = mixed;
   L.tau = p.TAU;
   {
     const float t2 = p.TAU * p.TAU;  // a float product, as the reference forms it
@@ -1264,7 +1263,7 @@ int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double 
     st = stream_charge(ctx, *pl, *p, d_nb, J_scratch, s, &J);
     if (st) return st;
   }
-  EventLandau L = landau_event(*p, T, em, ctx->path == FSM_PATH_MIXED);
+  EventLandau L = landau_event(*p, T, em);
   L.eta_zero = p->DIM_ETA == 1;  // the 2-D stream kernel writes the eta components as zeros
   L.out = d_out;
   if (J) {
@@ -1542,7 +1541,7 @@ int fsm_landau_device(fsm_ctx *ctx, const fsm_params *p, const float *d_T, float
   hipStream_t s = (hipStream_t)stream;
   CallOrder order(ctx, s);  // the timing events are the context's too
   FSM_TRY(order.begin());
-  EventLandau L = landau_event(*p, d_T, 0, ctx->path == FSM_PATH_MIXED);
+  EventLandau L = landau_event(*p, d_T, 0);
   L.fe = d_e;
   L.fu = d_u;
   L.fp = d_p;
@@ -1619,7 +1618,7 @@ int fsm_solve_eigen_system(fsm_ctx *ctx, const fsm_params *p, const float *const
   CallOrder order(ctx, io.s);
   FSM_TRY(order.begin());
   if ((st = io.put(T, 10, 0))) return st;
-  EventLandau L = landau_event(*p, io.base, 0, ctx->path == FSM_PATH_MIXED);
+  EventLandau L = landau_event(*p, io.base, 0);
   L.fe = io.base + 10 * io.D;
   L.fu = io.base + 11 * io.D;
   FSM_TRY(fsm_launch_landau(L, io.s));
@@ -2260,7 +2259,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     }
     int em = 0, jm = 0;
     const float *T = add_pass(ev, ev.d_e, false, cur, &em);
-    EventLandau L = landau_event(p, T, em, path == FSM_PATH_MIXED);
+    EventLandau L = landau_event(p, T, em);
     L.eta_zero = p.DIM_ETA == 1;  // the 2-D stream kernel writes the eta components as zeros
     if (!em) {
       // the batch's 2-D T (and J) rows sit at the padded pitch
@@ -2485,7 +2484,6 @@ int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_G
   if (!plan) return FSM_ERR_NO_PLAN;
   const Plan &pl = *plan;
   char *ws = (char *)ctx->noalloc_scratch;
-  const int mixed = ctx->path == FSM_PATH_MIXED;
   EventLandau L;
   if (!pl.is3d) {
     // the one-event kernels of fsm_run_device, with the spline planes, the
@@ -2495,7 +2493,7 @@ int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_G
     Event2D ev = event2d_in_workspace(pl, *p, d_e_GeV, ctx->path, v, cur, false);
     FSM_TRY(fsm_launch_spline2d(ev, s));
     FSM_TRY(fsm_launch_stream2d(ev, s));
-    L = landau_event(*p, ev.T, 0, mixed);
+    L = landau_event(*p, ev.T, 0);
     L.eta_zero = 1;
   } else {
     // source planes and eta-major T in the workspace
@@ -2506,7 +2504,7 @@ int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_G
     ev.t_plane = p->DIM;
     FSM_TRY(fsm_launch_src3d(ev, s));
     FSM_TRY(fsm_launch_stream3d(ev, s));
-    L = landau_event(*p, ev.T, 1, mixed);
+    L = landau_event(*p, ev.T, 1);
   }
   L.out = d_out;
   FSM_TRY(fsm_launch_landau(L, s));

@@ -323,7 +323,6 @@ struct EventLandau {
   int eos_type;
   int viscous;
   int fill_outputs;     // OUTPUTFORMAT is 2 or 3
-  int mixed;            // 1: FP32 Landau sweeps with FP64 for ill-conditioned cells
   int eta_zero;         // 1: T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} are zero by construction (a 2-D
                         // event's tensor from the stream kernel): the 3x3 Landau frame
   float tau;

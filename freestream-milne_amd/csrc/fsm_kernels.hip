@@ -724,8 +724,8 @@ __global__ __launch_bounds__(FSM3D_THREADS, 4) void k_stream3d_batch(const Event
 // ======================================================================
 // Landau matching + EoS + viscous parts + output pack
 // ======================================================================
-// Landau frame of one cell.  In the orthonormal frame (eta components scaled
-// by TAU) the matrix T^mu_nu = T^{mu rho} g_{rho nu} of
+// Landau frame of one cell by Jacobi sweeps.  In the orthonormal frame (eta
+// components scaled by TAU) the matrix T^mu_nu = T^{mu rho} g_{rho nu} of
 // src/LandauMatch.cpp:177-214 becomes A eta with A symmetric; its timelike
 // eigenvector is the frame in which A^{0i} = 0.  Each step is the exact
 // Lorentz boost in the (0,i) plane that zeroes A^{0i}:
@@ -740,10 +740,8 @@ __global__ __launch_bounds__(FSM3D_THREADS, 4) void k_stream3d_batch(const Event
 // reference's fallback applies (e = 1e-3, u = (1,0,0,0) written).
 // tol2: squared relative off-diagonal residual of convergence; loose2: the
 // residual still accepted after the last sweep.
-// Reciprocal square root for the Jacobi rotations: the hardware estimate in
-// FP32 (1 ulp), refined by two Newton steps in FP64 (each doubles the bits).
-// Arguments are positive and finite.
-__device__ __forceinline__ float jrsq(float x) { return __builtin_amdgcn_rsqf(x); }
+// Reciprocal square root in FP64: the hardware estimate refined by two
+// Newton steps (each doubles the bits).  Arguments are positive and finite.
 __device__ __forceinline__ double jrsq(double x) {
   double y = __builtin_amdgcn_rsq(x);
   double r = fma(-(x * y), y, 1.0);
@@ -752,17 +750,12 @@ __device__ __forceinline__ double jrsq(double x) {
   return fma(0.5 * y, r, y);
 }
 
-// `w` (optional): a velocity near the Landau frame's (spatial components in
-// the orthonormal frame), from which the sweeps start: A is first boosted by
-// B(-w), the pure boost to that velocity's rest frame, built from w with
-// gamma recomputed so that it is a Lorentz transformation to working
-// precision, and M starts at B(w).  The sweeps then only remove what is left
-// of A^{0i}; the frame they converge to is the same.
-// (2-D events, whose tensors from the stream kernel have zero eta
-// components, take the closed 3x3 form below instead: landau_cubic.)
+// These sweeps decide the cells where the characteristic quartic (below)
+// does not settle or fails a rule.  (2-D events, whose tensors from the
+// stream kernel have zero eta components, take the closed 3x3 form below:
+// landau_cubic.)
 template <typename R>
-__device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4],
-                             const R *w = nullptr) {
+__device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, float *e_out, float u_out[4]) {
   constexpr int N = 4;
   const R tau = (R)TAU;
   R A[N][N], M[N][N];
@@ -780,36 +773,6 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) M[i][j] = (i == j) ? (R)1 : (R)0;
-  if (w) {
-    const R v[4] = {(R)0, w[0], w[1], w[2]};
-    const R g = sqrt((R)1 + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
-    const R h = (R)1 / ((R)1 + g);
-    // B(w) = [[g, w^T], [w, I + w w^T / (1 + g)]]; B(-w) flips the 0i signs
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j)
-        M[i][j] = (i == 0 && j == 0) ? g : (i == 0) ? v[j] : (j == 0) ? v[i] : ((i == j ? (R)1 : (R)0) + h * v[i] * v[j]);
-    R C[N][N];  // A B(-w)
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j) {
-        R acc = (R)0;
-#pragma unroll
-        for (int k = 0; k < N; ++k) acc += A[i][k] * (((k == 0) != (j == 0)) ? -M[k][j] : M[k][j]);
-        C[i][j] = acc;
-      }
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = i; j < N; ++j) {
-        R acc = (R)0;
-#pragma unroll
-        for (int k = 0; k < N; ++k) acc += (((i == 0) != (k == 0)) ? -M[i][k] : M[i][k]) * C[k][j];
-        A[i][j] = A[j][i] = acc;
-      }
-  }
   auto off_diag = [&]() {
     return A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[0][3] * A[0][3];
   };
@@ -881,26 +844,108 @@ __device__ int landau_jacobi(const float t[10], float TAU, R tol2, R loose2, flo
 // (on sampled cells of a lumpy 2-D and a 3-D event its float results equal
 // those of a 1e-14 criterion; the eigenvalue error is quadratic in the
 // residual).
-__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4], const double *w = nullptr) {
-  (void)landau_jacobi<double>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out, w);
+__device__ void landau_fp64(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  (void)landau_jacobi<double>(t, TAU, 1.0e-24, 1.0e-12, e_out, u_out);
 }
 
-// FP32 stage of the mixed path.  Returns true when FP32 decides the result to
-// the parity tolerance; false for a failed or unconverged solve, fast flow
-// (u^tau > 1.5: the frame is conditioned like gamma^2) and cells whose e is a
-// small difference of large terms (kappa > 16: nearly null tensors in dilute
-// edge cells, where FP32 rounding in T is amplified by kappa).  Those cells
-// are solved again in FP64.
-__device__ bool landau_fp32_ok(const float t[10], float TAU, float *e_out, float u_out[4], int *st_out) {
-  const int st = landau_jacobi<float>(t, TAU, 1.0e-12f, 1.0e-10f, e_out, u_out);
-  *st_out = st;
-  // First-order sensitivity of e = T_{mu nu} u^mu u^nu to a relative change
-  // of every component: kappa = sum |u_mu| |T^{mu nu}| |u_nu| / e (>= u0^2 T^tt / e).
-  const float a0 = fabsf(u_out[0]), a1 = fabsf(u_out[1]), a2 = fabsf(u_out[2]), a3 = TAU * TAU * fabsf(u_out[3]);
-  const float S = a0 * a0 * fabsf(t[0]) + a1 * a1 * fabsf(t[4]) + a2 * a2 * fabsf(t[7]) + a3 * a3 * fabsf(t[9]) +
-                  2.0f * (a0 * (a1 * fabsf(t[1]) + a2 * fabsf(t[2]) + a3 * fabsf(t[3])) +
-                          a1 * (a2 * fabsf(t[5]) + a3 * fabsf(t[6])) + a2 * a3 * fabsf(t[8]));
-  return st == 1 && u_out[0] <= 1.5f && S <= 16.0f * *e_out;
+__device__ __forceinline__ double lrcp(double x);  // below, with landau_cubic
+
+// Landau frame of a 4x4 cell from the characteristic quartic (the eigenpair
+// that solveEigenSystem, src/LandauMatch.cpp:145-331, takes from a general
+// eigen-solver).  In the orthonormal frame (eta components times tau) S = T
+// is symmetric, g = diag(1, -1, -1, -1), and T g u = e u is (S - e g) w = 0
+// with w = g u.  p(l) = det(S - l g) = -l^4 + c3 l^3 + c2 l^2 + c1 l + c0,
+// whose coefficients are signed sums of the principal minors of S.  For a
+// tensor of free-streamed massless particles (a positive sum of null dyads)
+// the roots are e > 0 and -P_i <= 0, so e is the largest root, and
+// T^{tau tau} and T(n, n) at the velocity estimate n bound it from above;
+// Halley's iteration from there converges to it (cubically near the root).
+// w is any row of adj(S - e g) = c w w^T; row 0 is the longest (|w^0| =
+// u^tau > |u^i|) and needs only the 2x2 minors of rows 2 and 3.  S is first
+// scaled by a power of two near 1 / T^{tau tau} (exact), so no product of
+// up to six components leaves the float range.
+// FP64 (an FP64 FMA issues at the FP32 rate on this GPU).
+// Returns 1 when the frame is found and passes the reference's rules
+// (e > 0, u timelike, u^tau < 100); 0 when the iteration did not settle or a
+// rule fails, and the caller then decides the cell with the Jacobi sweeps.
+__device__ int landau_quartic(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  using R = double;
+  const R tau = (R)TAU;
+  const R t00 = (R)t[0];
+  if (!(t00 > (R)0) || !isfinite(t00)) return 0;
+  int ex;
+  (void)frexp(t00, &ex);
+  const R sc = ldexp((R)1, -ex);  // t00 * sc in [0.5, 1)
+  const R s00 = t00 * sc, s01 = (R)t[1] * sc, s02 = (R)t[2] * sc, s03 = (tau * (R)t[3]) * sc;
+  const R s11 = (R)t[4] * sc, s12 = (R)t[5] * sc, s13 = (tau * (R)t[6]) * sc;
+  const R s22 = (R)t[7] * sc, s23 = (tau * (R)t[8]) * sc;
+  const R s33 = (tau * tau * (R)t[9]) * sc;
+  // principal minors
+  const R q01 = s01 * s01, q02 = s02 * s02, q03 = s03 * s03, q12 = s12 * s12, q13 = s13 * s13, q23 = s23 * s23;
+  const R m01 = s00 * s11 - q01, m02 = s00 * s22 - q02, m03 = s00 * s33 - q03;
+  const R m12 = s11 * s22 - q12, m13 = s11 * s33 - q13, m23 = s22 * s33 - q23;
+  const R m123 = s11 * m23 - s22 * q13 - s33 * q12 + (R)2 * s12 * s23 * s13;
+  const R m023 = s00 * m23 - s22 * q03 - s33 * q02 + (R)2 * s02 * s23 * s03;
+  const R m013 = s00 * m13 - s11 * q03 - s33 * q01 + (R)2 * s01 * s13 * s03;
+  const R m012 = s00 * m12 - s11 * q02 - s22 * q01 + (R)2 * s01 * s12 * s02;
+  // det S from the 2x2 minors of rows 0, 1 and of rows 2, 3
+  const R a01 = s00 * s11 - s01 * s01, a02 = s00 * s12 - s02 * s01, a03 = s00 * s13 - s03 * s01;
+  const R a12 = s01 * s12 - s02 * s11, a13 = s01 * s13 - s03 * s11, a23 = s02 * s13 - s03 * s12;
+  const R b01 = s02 * s13 - s12 * s03, b02 = s02 * s23 - s22 * s03, b03 = s02 * s33 - s23 * s03;
+  const R b12 = s12 * s23 - s22 * s13, b13 = s12 * s33 - s23 * s13, b23 = s22 * s33 - s23 * s23;
+  const R c0 = a01 * b23 - a02 * b13 + a03 * b12 + a12 * b03 - a13 * b02 + a23 * b01;
+  const R c1 = m012 + m013 + m023 - m123;
+  const R c2 = m01 + m02 + m03 - m12 - m13 - m23;
+  const R c3 = s00 - s11 - s22 - s33;
+  // start: T(n, n) at n = gamma (1, v), v = T^{0i} / (T^{00} + tr/3)
+  const R rd = lrcp(s00 + (s11 + s22 + s33) * (R)(1.0 / 3.0));
+  const R vx = s01 * rd, vy = s02 * rd, vz = s03 * rd;
+  const R g2 = lrcp((R)1 - vx * vx - vy * vy - vz * vz);
+  const R tnn = g2 * (s00 - (R)2 * (vx * s01 + vy * s02 + vz * s03) + vx * vx * s11 + vy * vy * s22 +
+                      vz * vz * s33 + (R)2 * (vx * vy * s12 + vx * vz * s13 + vy * vz * s23));
+  R lam = fmin(s00, tnn);  // fmin: T^{00} where the estimate is not a number
+  // After a step below tol the remaining error is of order tol^3: rounding.
+  const R tol = (R)1.0e-10;
+  bool conv = false;
+  for (int it = 0; it < 16; ++it) {
+    const R p = (((c3 - lam) * lam + c2) * lam + c1) * lam + c0;
+    const R dp = (((R)3 * c3 - (R)4 * lam) * lam + (R)2 * c2) * lam + c1;
+    const R d2 = ((R)6 * c3 - (R)12 * lam) * lam + (R)2 * c2;
+    const R st = (R)2 * p * dp * lrcp((R)2 * dp * dp - p * d2);
+    lam -= st;
+    if (!(fabs(st) > tol * fabs(lam))) {
+      conv = fabs(st) <= tol * fabs(lam);  // false for a NaN
+      break;
+    }
+  }
+  if (!conv || !(lam > (R)0)) return 0;
+  // row 0 of adj(S - lam g): rows 2, 3 of B = S - lam g and their 2x2 minors
+  const R b11 = s11 + lam, b22 = s22 + lam, b33 = s33 + lam;
+  const R n01 = s02 * s13 - s12 * s03, n02 = s02 * s23 - b22 * s03, n03 = s02 * b33 - s23 * s03;
+  const R n12 = s12 * s23 - b22 * s13, n13 = s12 * b33 - s23 * s13, n23 = b22 * b33 - s23 * s23;
+  const R w0 = b11 * n23 - s12 * n13 + s13 * n12;
+  const R w1 = s01 * n23 - s12 * n03 + s13 * n02;    // -C01 = u^x direction
+  const R w2 = -(s01 * n13 - b11 * n03 + s13 * n01);  // -C02
+  const R w3 = s01 * n12 - b11 * n02 + s12 * n01;    // -C03
+  const R ml = w0 * w0 - (w1 * w1 + w2 * w2 + w3 * w3);
+  if (!(ml > (R)0) || !isfinite(ml)) return 0;
+  R f = jrsq(ml);
+  if (w0 < (R)0) f = -f;
+  const R u0 = w0 * f;
+  if (!(u0 < (R)100)) return 0;
+  *e_out = (float)ldexp(lam, ex);
+  u_out[0] = (float)u0;
+  u_out[1] = (float)(w1 * f);
+  u_out[2] = (float)(w2 * f);
+  u_out[3] = (float)(w3 * f / tau);
+  return 1;
+}
+
+// Landau frame of a 4x4 cell, both paths: the quartic, and the Jacobi sweeps
+// where the quartic does not settle or a rule fails (those decide the
+// reference's fallback).
+__device__ void landau_frame4(const float t[10], float TAU, float *e_out, float u_out[4]) {
+  if (!landau_quartic(t, TAU, e_out, u_out)) landau_fp64(t, TAU, e_out, u_out);
 }
 
 // Landau frame of a 2-D event's tensor (eta components zero), FP64.  With
@@ -1046,86 +1091,22 @@ struct CellOut {
   float nb, V[4];  // baryon extension (with ev.J)
 };
 
-// One workgroup's list of cells that the mixed path re-solves in FP64.
-struct LandauShared {
-  int n64;
-  int who[256];   // threads whose cell needs FP64
-  int tidx[256];  // T index of each listed thread's cell
-  int warm[256];  // 1: the FP32 frame converged and seeds the FP64 sweeps
-  float e[256], u[4][256];  // FP32 frame in, FP64 result out
-};
-
-// Landau frame of the workgroup's cells (one per thread, `live` marks real
-// cells; every thread of the workgroup calls this).  N = 3 (2-D events): the
-// FP64 cubic per cell on both paths.  N = 4: FP32 path, FP64 Jacobi for
-// every cell; mixed path, FP32 Jacobi for every cell, then the cells that
-// FP32 cannot decide are gathered into a dense list and re-solved in FP64 by
-// the first threads of the workgroup, so FP64 work is not spread over
-// partly-idle waves; a converged FP32 frame is the FP64 sweeps' starting
-// point (about half the sweeps of a cold start).
-template <int N>
-__device__ void landau_group_solve(const EventLandau &ev, bool live, long tidx, const float t[10], float *e,
-                                   float u[4], LandauShared &S) {
-  if constexpr (N == 3) {
-    // 2-D events, both paths: the FP64 cubic (no workgroup list needed)
-    (void)ev;
-    (void)tidx;
-    (void)S;
-    if (live) landau_cubic(t, e, u);
-  } else {
-    if (!ev.mixed) {
-      if (live) landau_fp64(t, ev.tau, e, u);
-      return;
-    }
-    bool need = false;
-    int st = 0;
-    if (live) need = !landau_fp32_ok(t, ev.tau, e, u, &st);
-    __syncthreads();  // the previous group's list is no longer read
-    if (threadIdx.x == 0) S.n64 = 0;
-    __syncthreads();
-    if (need) {
-      const int k = atomicAdd(&S.n64, 1);
-      S.who[k] = threadIdx.x;
-      S.tidx[threadIdx.x] = (int)tidx;
-      S.warm[threadIdx.x] = st;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) S.u[c][threadIdx.x] = u[c];
-    }
-    __syncthreads();
-    const int n = S.n64;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      const int w = S.who[k];
-      const long ti = S.tidx[w];
-      float tt[10], e64, u64[4];
-#pragma unroll
-      for (int c = 0; c < 10; ++c) tt[c] = ev.T[c * ev.t_plane + ti];
-      // the FP32 frame (orthonormal components) seeds the FP64 sweeps
-      const double w3[3] = {(double)S.u[1][w], (double)S.u[2][w], (double)ev.tau * (double)S.u[3][w]};
-      landau_fp64(tt, ev.tau, &e64, u64, S.warm[w] ? w3 : nullptr);
-      S.e[w] = e64;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) S.u[c][w] = u64[c];
-    }
-    __syncthreads();
-    if (need) {
-      *e = S.e[threadIdx.x];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) u[c] = S.u[c][threadIdx.x];
-    }
-  }
-}
-
 // Landau frame, EoS, bulk and shear parts of the cell whose T components sit
-// at index tidx of ev.T's planes (workgroup-uniform call, see above).
+// at index tidx of ev.T's planes.  N = 3 (2-D events, eta components zero):
+// the 3x3 frame from the characteristic cubic; N = 4: the 4x4 frame from the
+// quartic.  Both in FP64 on both paths.
 template <int N>
-__device__ void landau_cell_compute(const EventLandau &ev, bool live, long tidx, CellOut &r, LandauShared &S) {
+__device__ void landau_cell_compute(const EventLandau &ev, bool live, long tidx, CellOut &r) {
 #pragma clang fp contract(off)
+  if (!live) return;
   float t[10];
 #pragma unroll
-  for (int c = 0; c < 10; ++c) t[c] = (live && (N == 4 || (c != 3 && c != 6 && c != 8 && c != 9))) ? ev.T[c * ev.t_plane + tidx] : 0.0f;
-  float e = 0.0f, u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  landau_group_solve<N>(ev, live, tidx, t, &e, u, S);
-  if (!live) return;
+  for (int c = 0; c < 10; ++c) t[c] = (N == 4 || (c != 3 && c != 6 && c != 8 && c != 9)) ? ev.T[c * ev.t_plane + tidx] : 0.0f;
+  float e, u[4];
+  if constexpr (N == 3)
+    landau_cubic(t, &e, u);
+  else
+    landau_frame4(t, ev.tau, &e, u);
   const float TAU = ev.tau;
   const float p = eos_pressure(e, ev.eos_type);
   float Pi = 0.0f;
@@ -1191,7 +1172,7 @@ __device__ void cell_store_float(const EventLandau &ev, long is, const CellOut &
 // One cell per thread, reference index is (T read through the layout map);
 // workgroup-uniform call.
 template <int N>
-__device__ void landau_cell_out(const EventLandau &ev, bool live, long is, LandauShared &S) {
+__device__ void landau_cell_out(const EventLandau &ev, bool live, long is) {
   long tidx = is;
   if (ev.t_eta_major) {
     const long xy = is / ev.neta;
@@ -1202,7 +1183,7 @@ __device__ void landau_cell_out(const EventLandau &ev, bool live, long is, Landa
     tidx = ix * ev.t_row + (is - ix * ev.t_ny);
   }
   CellOut r;
-  landau_cell_compute<N>(ev, live, tidx, r, S);
+  landau_cell_compute<N>(ev, live, tidx, r);
   if (!live) return;
   if (ev.out) {
 #pragma unroll
@@ -1220,17 +1201,16 @@ __device__ void landau_cell_out(const EventLandau &ev, bool live, long is, Landa
 // cells.  T is read along the transverse index (coalesced in the eta-major
 // layout); the outputs go through LDS and leave along eta, the fast index of
 // the reference layout, so both sides move whole cache-line runs.
-__device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds, LandauShared &S) {
+__device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds) {
   const int xl = threadIdx.x % FSML_TX, el = threadIdx.x / FSML_TX;
   const long xy = (long)xt * FSML_TX + xl;
   const int ie = et * FSML_TE + el;
   const bool live = xy < ev.nxy && ie < ev.neta;
   const int np = ev.J ? FSM_NUM_OUTPUTS + 5 : FSM_NUM_OUTPUTS;
   CellOut r;
-  // On the mixed path this starts with a barrier; on the FP32 path it has
-  // none, so the trailing barrier below is what keeps the staged outputs of
-  // one tile from being overwritten by the next tile of the same workgroup.
-  landau_cell_compute<4>(ev, live, (long)ie * ev.nxy + xy, r, S);
+  // The trailing barrier below keeps the staged outputs of one tile from
+  // being overwritten by the next tile of the same workgroup.
+  landau_cell_compute<4>(ev, live, (long)ie * ev.nxy + xy, r);
   if (live) {
     const int li = xl * FSML_PITCH + el;
 #pragma unroll
@@ -1281,23 +1261,21 @@ __global__ __launch_bounds__(256) void k_stage_cells(StageCells a) {
 
 template <int N>
 __global__ __launch_bounds__(256) void k_landau_one(EventLandau ev) {
-  __shared__ LandauShared S;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long base = (long)blockIdx.x * blockDim.x; base < ev.dim; base += stride) {
     const long is = base + threadIdx.x;
-    landau_cell_out<N>(ev, is < ev.dim, is, S);
+    landau_cell_out<N>(ev, is < ev.dim, is);
   }
 }
 
 __global__ __launch_bounds__(256) void k_landau3d_one(EventLandau ev) {
   extern __shared__ float lds_out[];
-  __shared__ LandauShared S;
   // eta tiles fastest: the workgroups running together cover whole eta runs
   // of a few transverse cells, i.e. contiguous stretches of every output
   // plane (eta is the fast index of the reference layout)
   const int etiles = (ev.neta + FSML_TE - 1) / FSML_TE;
   const int b = xcd_order(blockIdx.x, gridDim.x);
-  landau3d_tile(ev, b / etiles, b % etiles, lds_out, S);
+  landau3d_tile(ev, b / etiles, b % etiles, lds_out);
 }
 
 // Batched: item = (event, cell offset, cell count, 0) for 2-D events, or
@@ -1305,15 +1283,14 @@ __global__ __launch_bounds__(256) void k_landau3d_one(EventLandau ev) {
 __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restrict__ evs,
                                                       const int4 *__restrict__ items, int n_items) {
   extern __shared__ float lds_out[];
-  __shared__ LandauShared S;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int4 item = fsm_sload(items + xcd_order(it, n_items));
     const EventLandau ev = fsm_sload(evs + item.x);
     if (item.w) {
-      landau3d_tile(ev, item.y, item.z, lds_out, S);
+      landau3d_tile(ev, item.y, item.z, lds_out);
     } else {
       for (int base = 0; base < item.z; base += blockDim.x)
-        landau_cell_out<4>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
+        landau_cell_out<4>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x);
     }
   }
 }
@@ -1323,12 +1300,11 @@ __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restr
 // 3x3 Landau frame.
 __global__ __launch_bounds__(256) void k_landau2d_batch(const EventLandau *__restrict__ evs,
                                                         const int4 *__restrict__ items, int n_items) {
-  __shared__ LandauShared S;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int4 item = fsm_sload(items + xcd_order(it, n_items));
     const EventLandau ev = fsm_sload(evs + item.x);
     for (int base = 0; base < item.z; base += blockDim.x)
-      landau_cell_out<3>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x, S);
+      landau_cell_out<3>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x);
   }
 }
 

@@ -1,6 +1,6 @@
 """Mixed-precision path (FSM_PATH_MIXED): the 2-D quadrature on FP16 MFMA with
-hi+lo split knots and weights, and FP32 Landau sweeps with FP64 for dilute
-and high-gamma cells (in 3-D the quadrature stays on the FP32 stream kernel).  Same bar as the FP32 path: T^{mu nu} within 1e-4 of
+hi+lo split knots and weights (in 3-D the quadrature stays on the FP32 stream
+kernel); the Landau frame is FP64 on both paths.  Same bar as the FP32 path: T^{mu nu} within 1e-4 of
 max T^{tau tau} against the reference float path (oracle) and, in 2-D,
 against the FP64 oracle on cells whose samples all stay inside the grid
 (float and FP64 keep different samples at the mask boundary); e, u, pi, Pi
@@ -192,8 +192,7 @@ def test_mixed_batch_matches_single_events(mixed_ctx):
 
 @pytest.mark.parametrize("name", ["C1", "hotspots_spikes"])
 def test_mixed_landau_on_identical_input(mixed_ctx, name):
-    """Stage B alone on the mixed path (FP32 sweeps, FP64 where the cell is
-    ill-conditioned or fast) against the oracle's FP64 eigen-solve on the
+    """Stage B alone on the mixed path against the oracle's FP64 eigen-solve on the
     oracle's float T^{mu nu}, every cell with e > 1e-3 e_max."""
     import torch
 
