@@ -1078,14 +1078,14 @@ int get_plan_shared(fsm_ctx *ctx, const fsm_params &p, std::shared_ptr<Plan> *ou
 // Batched device time per unit of work on MI355X, in picoseconds, from the
 // measured stage times of the 261^2 lumpy event (64-event batches: stream
 // 14.4 / 8.5 us for 13.6 M samples, spline 1.5 us and Landau 3.0 us for 68 k
-// cells) and of the 131^2 x 101 event (stream 0.47 ms for 6.8e8 evaluated
-// samples, Landau 0.14 ms for 1.73 M cells): 2-D stream per (cell, phi)
-// sample on each path, 3-D stream per evaluated sample, spline or source
-// planes and Landau per cell.
+// cells), of the 131^2 x 101 event (stream 0.47 ms for 6.8e8 evaluated
+// samples) and of a batch of four 131^2 x 41..101 events (Landau 0.27 ms for
+// 4.87 M cells): 2-D stream per (cell, phi) sample on each path, 3-D stream
+// per evaluated sample, spline or source planes and Landau per cell.
 const double kCostStream2dPs[2] = {1.06, 0.62};  // FP32, mixed
 const double kCostStream3dPs = 0.70;
 const double kCostPlanes2dPs = 22.0, kCostPlanes3dPs = 17.0;
-const double kCostLandau2dPs = 36.0, kCostLandau3dPs = 81.0;
+const double kCostLandau2dPs = 36.0, kCostLandau3dPs = 55.0;
 
 // Estimated device time of one event inside a batch, microseconds.
 double event_cost_us(const Plan &pl, const fsm_params &p, int path, bool baryon) {
@@ -2273,7 +2273,7 @@ int batch_build(fsm_batch *b, hipStream_t s) {
       L.out_b = ev.d_out + (size_t)FSM_NUM_OUTPUTS * p.DIM;
     }
     el.push_back(L);
-    if (em && p.DIM_ETA > 1) {
+    if (em) {
       // 3-D: eta x transverse tiles of the LDS-transposed Landau form
       const int nxy = p.DIM_X * p.DIM_Y;
       // eta tiles fastest, as in the one-event launch (contiguous output stretches)
@@ -2281,8 +2281,8 @@ int batch_build(fsm_batch *b, hipStream_t s) {
         for (int et = 0; et < (p.DIM_ETA + FSML_TE - 1) / FSML_TE; ++et) ld.push_back(make_int4(li, xt, et, 1));
       b->lds_l = std::max(b->lds_l, fsm_landau_lds_bytes(ev.d_nb != nullptr));
     } else {
-      for (int c = 0; c < p.DIM; c += kLandauChunk)
-        (L.eta_zero ? ld2 : ld).push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
+      // 2-D (DIM_ETA = 1, eta components zero): cell chunks of the 3x3 form
+      for (int c = 0; c < p.DIM; c += kLandauChunk) ld2.push_back(make_int4(li, c, std::min(kLandauChunk, p.DIM - c), 0));
     }
   }
   {

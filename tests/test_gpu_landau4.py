@@ -135,6 +135,8 @@ def test_landau4_known_frames(path):
 def _case3d(name):
     if name == "small3d":
         return W.small_3d(), "tan"
+    if name == "small3d_int":  # DTAU/DX integer: samples on grid lines, nearly null single-beam cells
+        return W.small_3d(nx=33, neta=17, nrap=7, nphi=16, dtau=0.5, seed=9), "tan"
     if name == "rect3d":
         from test_gpu_mixed import _case as mixed_case
 
@@ -145,7 +147,7 @@ def _case3d(name):
 
 
 @pytest.mark.parametrize("path", ["fp32", "mixed"])
-@pytest.mark.parametrize("name", ["small3d", "rect3d", "c2_71_gh"])
+@pytest.mark.parametrize("name", ["small3d", "small3d_int", "rect3d", "c2_71_gh"])
 def test_landau4_on_oracle_tensors(name, path):
     """Stage B of 3-D events on identical input: the oracle's float T through
     fsm_landau_device, every cell with e > 1e-3 e_max."""
@@ -166,3 +168,40 @@ def test_landau4_on_oracle_tensors(name, path):
           % (int(m.sum()), ee, ue, mism), flush=True)
     assert ee < BAR[path] and ue < BAR[path]
     assert mism == 0
+
+
+@pytest.mark.parametrize("path", ["fp32", "mixed"])
+@pytest.mark.parametrize("name", ["small3d", "small3d_int", "rect3d"])
+def test_landau4_frame_on_gpu_tensor(name, path):
+    """The run path's 3-D Landau stage (eta-major tiles) on the GPU's own
+    T^{mu nu}: e and u of every cell with e > 1e-3 e_max against the oracle's
+    FP64 eigen-solve of the same tensor, and the reference's fallback
+    decision on every cell (no conditioning exclusion)."""
+    import torch
+
+    (p, e), _ = _case3d(name)
+    ctx = _ctx(path)
+    q = fsm_amd.params_finalize(p, e)
+    D = q.DIM
+    d_e = torch.from_numpy(np.ascontiguousarray(e, dtype=np.float32)).cuda()
+    d_T = torch.empty(10 * D, dtype=torch.float32, device="cuda")
+    d_out = torch.empty(17 * D, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream()
+    ctx.stream_tmunu_device(q, d_e.data_ptr(), d_T.data_ptr(), s)
+    ctx.run_device(q, d_e.data_ptr(), d_out.data_ptr(), s)
+    torch.cuda.synchronize()
+    ctx.close()
+    T = d_T.cpu().numpy().reshape(10, D)
+    out = d_out.cpu().numpy().reshape(17, D)
+    hb = np.float64(np.float32(0.197326938))
+    eg = out[0] / hb  # (double)e * hbarc is exact, so this is the float e
+    ug = out[2:6].T
+    eo, uo = O.landau(T.astype(np.float64).T, np.float32(q.TAU))
+    m = eo > 1e-3 * eo.max()
+    ee = (np.abs(eg - eo)[m] / eo[m]).max()
+    ue = (np.abs(ug - uo)[m].max(axis=1) / uo[m, 0]).max()
+    fo, fg = _fallback(eo, uo), _fallback(eg, ug)
+    print(name, path, "3-D frame on the GPU's T: cells %d, e rel err %.3g, u rel err %.3g, fallbacks %d, mismatches %d"
+          % (int(m.sum()), ee, ue, int(fo.sum()), int((fo != fg).sum())), flush=True)
+    assert ee < BAR[path] and ue < BAR[path]
+    assert np.array_equal(fo, fg)
