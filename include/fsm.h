@@ -130,8 +130,8 @@ enum {
   FSM_PATH_MIXED = 1  /* 2-D: interpolation + quadrature as one contraction
                          on FP16 MFMA (knots and weights split hi+lo, FP32
                          accumulation), FP32 sample loop where the spline can
-                         undershoot; Landau in FP32 with FP64 for dilute and
-                         high-gamma cells.  No counterpart in the reference. */
+                         undershoot; the same FP64 Landau frame as the FP32
+                         path.  No counterpart in the reference. */
 };
 int fsm_set_path(fsm_ctx *ctx, int path);
 
