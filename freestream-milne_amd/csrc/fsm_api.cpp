@@ -460,8 +460,8 @@ void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char>
   for (size_t o = 0; o < no; ++o)
     bmat_lanes([&](int g, int c, int m) { return W[o][(g * 4 + c) * 6 + m]; }, offb[o].data());
   // greedy groups along the phi order of first appearance; a group's LDS
-  // image is its knot box, one packed B operand (64 x 8 B) and one record
-  // (16 B) per MFMA of either form
+  // image is its knot box and one record (16 B) per MFMA of either form (the
+  // packed B operands are read from global memory)
   const int TX = fsm_stream2d_tile_w(0), TY = fsm_stream2d_tile_h(0);
   t.lds4 = 0;
   t.rbx0 = t.rby0 = 1 << 30;
@@ -481,8 +481,7 @@ void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char>
       const int bx0 = std::min(G.bx0, t.offs[e].x), bx1 = std::max(G.bx1, t.offs[e].x + 1);
       const int by0 = std::min(G.by0, t.offs[e].y), by1 = std::max(G.by1, t.offs[e].y + 1);
       const int np = (cover(o, e + 1, bx1, nullptr) + 1) / 2;
-      if ((TX + bx1 - bx0) * ((TY + by1 - by0 + 15) & ~15) + (FSM_BMAT_U4 + 1) * ((int)(e + 1 - o) + np) >
-          budget4)
+      if ((TX + bx1 - bx0) * ((TY + by1 - by0 + 15) & ~15) + ((int)(e + 1 - o) + np) > budget4)
         break;
       G.bx0 = bx0;
       G.bx1 = bx1;
@@ -495,7 +494,7 @@ void build_mfma_tables(const std::vector<Stencil2D> &st, const std::vector<char>
     const int np = ((int)dom.size() + 1) / 2;
     // knot box columns padded to 16 records (TY rows + halo, rounded up)
     const int LYP = (TY + G.by1 - G.by0 + 15) & ~15;
-    t.lds4 = std::max(t.lds4, (TX + G.bx1 - G.bx0) * LYP + (FSM_BMAT_U4 + 1) * (G.j1 - G.j0 + np));
+    t.lds4 = std::max(t.lds4, (TX + G.bx1 - G.bx0) * LYP + (G.j1 - G.j0 + np));
     // the group's block: its offsets' records and B operands, then its pairs'
     G.pad0 = (int)recs.size();
     G.pad1 = np;

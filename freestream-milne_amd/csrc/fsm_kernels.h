@@ -41,6 +41,7 @@ hipError_t fsm_launch_src3d_batch(const Event3D *evs, const int4 *items, int n, 
 hipError_t fsm_launch_stream3d_batch(const Event3D *evs, const int4 *items, int n, size_t lds_bytes, hipStream_t s);
 // Dynamic LDS of the 3-D Landau tiles (17 output planes, 22 with baryon).
 size_t fsm_landau_lds_bytes(int baryon);
+// 3-D items (event, transverse tile, eta tile, 1) of events with eta-major T.
 hipError_t fsm_launch_landau_batch(const EventLandau *evs, const int4 *items, int n, size_t lds_bytes,
                                    hipStream_t s);
 // 2-D items of events with eta_zero set (the 3x3 Landau frame).

@@ -1278,20 +1278,15 @@ __global__ __launch_bounds__(256) void k_landau3d_one(EventLandau ev) {
   landau3d_tile(ev, b / etiles, b % etiles, lds_out);
 }
 
-// Batched: item = (event, cell offset, cell count, 0) for 2-D events, or
-// (event, transverse tile, eta tile, 1) for 3-D events (eta-major T).
+// Batched 3-D events (eta-major T): item = (event, transverse tile, eta
+// tile, 1).
 __global__ __launch_bounds__(256) void k_landau_batch(const EventLandau *__restrict__ evs,
                                                       const int4 *__restrict__ items, int n_items) {
   extern __shared__ float lds_out[];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int4 item = fsm_sload(items + xcd_order(it, n_items));
     const EventLandau ev = fsm_sload(evs + item.x);
-    if (item.w) {
-      landau3d_tile(ev, item.y, item.z, lds_out);
-    } else {
-      for (int base = 0; base < item.z; base += blockDim.x)
-        landau_cell_out<4>(ev, base + (int)threadIdx.x < item.z, (long)item.y + base + threadIdx.x);
-    }
+    landau3d_tile(ev, item.y, item.z, lds_out);
   }
 }
 

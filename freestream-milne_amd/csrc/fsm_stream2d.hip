@@ -327,13 +327,34 @@ __device__ __forceinline__ float sample_correction(const Event2D &ev, const Sten
 // halves 4-7 so that both knot halves meet both weight halves.  `b` is the
 // operand for cell columns sharing the lane's corner roles, `bs` for the odd
 // columns (B'[lane] = B[lane ^ 16]: ds_swizzle bit mode and 0x1f, xor 0x10).
-__device__ __forceinline__ void bmat_expand(const uint4 *bl, int m, int lane, fsm_half8 &b, fsm_half8 &bs) {
-  const uint2 v = reinterpret_cast<const uint2 *>(bl + m * FSM_BMAT_U4)[lane];
+__device__ __forceinline__ void bmat_expand(uint2 v, fsm_half8 &b, fsm_half8 &bs) {
   const uint2 w = make_uint2((unsigned)__builtin_amdgcn_ds_swizzle((int)v.x, 0x401F),
                              (unsigned)__builtin_amdgcn_ds_swizzle((int)v.y, 0x401F));
   b = __builtin_bit_cast(fsm_half8, make_uint4(v.x, v.y, v.x, v.y));
   bs = __builtin_bit_cast(fsm_half8, make_uint4(w.x, w.y, w.x, w.y));
 }
+
+// The packed B operands of a group's MFMAs stay in global memory (L2: every
+// tile of an event reads the same few tens of KiB); lane l's 8 bytes of
+// MFMA m are bg[64 m + l].  They are loaded two MFMA records ahead of their
+// use, so the loads are in flight during the previous records' MFMAs, and
+// they take no LDS, which leaves the group's LDS image to its knot box.
+struct BPrefetch {
+  const uint2 *bg;
+  uint2 v0, v1;
+  int n;
+  __device__ __forceinline__ BPrefetch(const uint2 *bg_, int n_, int lane) : bg(bg_ + lane), n(n_) {
+    v0 = n > 0 ? fsm_gload(bg) : make_uint2(0u, 0u);
+    v1 = n > 1 ? fsm_gload(bg + 64) : make_uint2(0u, 0u);
+  }
+  // B of record o (records are taken in order); starts the load of o + 2
+  __device__ __forceinline__ uint2 next(int o) {
+    const uint2 v = v0;
+    v0 = v1;
+    if (o + 2 < n) v1 = fsm_gload(bg + 64 * (o + 2));
+    return v;
+  }
+};
 
 // The MFMA loop over the knot offsets of one LDS stage, for the wave's XW
 // cell columns x 2 row blocks.  Adjacent cell columns share a knot column:
@@ -349,19 +370,20 @@ __device__ __forceinline__ void bmat_expand(const uint4 *bl, int m, int lane, fs
 // out of range skip their MFMAs; both tests are wave-uniform.  A second row
 // block with no live cell (y0 + 16 >= ny) is skipped in both forms.
 template <int XW, bool MASKED>
-__device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
+__device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint2 *bg,
                                              const int4 *orec, int no, int lane_base, int LYP, int gx, int r,
                                              int xw0, int y0, int nx, int ny) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
   const int nyb = y0 + 16 < ny ? 2 : 1;  // row blocks holding live cells
   no = __builtin_amdgcn_readfirstlane(no);
+  BPrefetch bq(bg, no, lane);
   for (int o = 0; o < no; ++o) {
     const int4 rec = orec[o];
     const int ox = __builtin_amdgcn_readfirstlane(rec.x), oy = __builtin_amdgcn_readfirstlane(rec.y);
     const int delta = __builtin_amdgcn_readfirstlane(rec.z);
     fsm_half8 b0, b1;
-    bmat_expand(bl, o, lane, b0, b1);
+    bmat_expand(bq.next(o), b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
       if (yb >= nyb) break;
@@ -415,17 +437,18 @@ __device__ __forceinline__ void mfma_offsets(fsm_float4v (&acc)[XW][2], const ui
 // unchanged.  Used where every cell of the wave keeps every sample of the
 // group (no masks), so merging samples of different offsets is exact.
 template <int XW>
-__device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint4 *bl,
+__device__ __forceinline__ void mfma_pairs(fsm_float4v (&acc)[XW][2], const uint4 *lh, const uint2 *bg,
                                            const int4 *prec, int np, int lane_base, int LYP, int gx, int gy,
                                            int nyb) {
   const int lane = threadIdx.x & 63;
   constexpr int NR = XW / 2 + 1;
   np = __builtin_amdgcn_readfirstlane(np);
+  BPrefetch bq(bg, np, lane);
   for (int o = 0; o < np; ++o) {
     const int4 rec = prec[o];
     const int delta = gy ? rec.y : rec.x;
     fsm_half8 b0, b1;
-    bmat_expand(bl, o, lane, b0, b1);
+    bmat_expand(bq.next(o), b0, b1);
 #pragma unroll
     for (int yb = 0; yb < 2; ++yb) {
       if (yb >= nyb) break;
@@ -458,14 +481,14 @@ __device__ const uint4 fsm_zero_knot = {0u, 0u, 0u, 0u};
 
 // LDS image of one offset group, copied straight from global memory (16 B
 // per lane, a wave fills 64 consecutive slots): the knot box [LX][LYP]
-// (knots off the grid and rows LY..LYP-1 read as zero), then the packed B
-// operands of the group's MFMAs of both forms, then their records.  Wave w
-// fills chunks w, w + 4, ... of each part.
+// (knots off the grid and rows LY..LYP-1 read as zero), then the records of
+// the group's MFMAs of both forms.  Wave w fills chunks w, w + 4, ... of each
+// part.
 __device__ __forceinline__ void group_dma(const Event2D &ev, const StencilGroup &G, uint4 *lh, int x0, int y0,
                                           int TX, int TY, int wave, int lane) {
   const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
   const int n = LX * LYP;
-  const int no = G.j1 - G.j0 + G.pad1, nw = no * FSM_BMAT_U4;
+  const int no = G.j1 - G.j0 + G.pad1;
   const int bx = x0 + G.bx0, by = y0 + G.by0;
   // slot s = 64 c + lane at box (lx, ly); s advances by 256 per chunk step
   const int s0 = wave * 64 + lane;
@@ -483,11 +506,7 @@ __device__ __forceinline__ void group_dma(const Event2D &ev, const StencilGroup 
       ++lx;
     }
   }
-  uint4 *bl = lh + n;
-  const uint4 *bsrc = ev.bmat + (long)G.pad0 * FSM_BMAT_U4;
-  for (int c = wave; c * 64 < nw; c += 4)
-    if (c * 64 + lane < nw) fsm_lds_dma16(bsrc + c * 64 + lane, bl + c * 64);
-  uint4 *rl = bl + nw;
+  uint4 *rl = lh + n;
   const uint4 *rsrc = reinterpret_cast<const uint4 *>(ev.moffs + G.pad0);
   for (int c = wave; c * 64 < no; c += 4)
     if (c * 64 + lane < no) fsm_lds_dma16(rsrc + c * 64 + lane, rl + c * 64);
@@ -534,8 +553,9 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     // knot columns c and c+1 do not collide in the LDS banks
     const int LX = TX + G.bx1 - G.bx0, LY = TY + G.by1 - G.by0, LYP = (LY + 15) & ~15;
     const int no = G.j1 - G.j0, np = G.pad1;
-    uint4 *bl = lh + LX * LYP;  // [no + np][FSM_BMAT_U4] packed weights: offsets, then pair MFMAs
-    int4 *orec = reinterpret_cast<int4 *>(bl + (no + np) * FSM_BMAT_U4);  // offsets (ox, oy, box delta, -), pairs (delta gy=0, gy=1, -, -)
+    // packed B operands in global memory: offsets, then pair MFMAs
+    const uint2 *bg = reinterpret_cast<const uint2 *>(ev.bmat + (long)G.pad0 * FSM_BMAT_U4);
+    int4 *orec = reinterpret_cast<int4 *>(lh + LX * LYP);  // offsets (ox, oy, box delta, -), pairs (delta gy=0, gy=1, -, -)
     __syncthreads();  // every wave is done with the previous group's image
     group_dma(ev, G, lh, x0, y0, TX, TY, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's copies have landed in LDS
@@ -550,10 +570,10 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     // waves whose columns hold no live cell (the last tile column) skip the group
     if (xw0 < nx) {
       if (interior)
-        mfma_pairs<XW>(acc, lh, bl + no * FSM_BMAT_U4, orec + no, np, lane_base - gy, LYP, gx, gy,
+        mfma_pairs<XW>(acc, lh, bg + no * 64, orec + no, np, lane_base - gy, LYP, gx, gy,
                        y0 + 16 < ny ? 2 : 1);
       else
-        mfma_offsets<XW, true>(acc, lh, bl, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
+        mfma_offsets<XW, true>(acc, lh, bg, orec, no, lane_base, LYP, gx, r, xw0, y0, nx, ny);
     }
   }
   // moments to LDS: Tt[m][lx*TY + ly]
