@@ -998,23 +998,28 @@ __device__ int landau_cubic(const float t[10], float *e_out, float u_out[4]) {
   const bool use_p = np >= nq && np >= ns, use_q = !use_p && nq >= ns;
   const double v0 = use_p ? p0 : use_q ? q0 : s0, v1 = use_p ? p1 : use_q ? q1 : s1, v2 = use_p ? p2 : use_q ? q2 : s2;
   const double ml = v0 * v0 - (v1 * v1 + v2 * v2);
+  // the reference's fallback unless the frame passes its rules; every output
+  // is written once, after the branch (no conditionally written array)
+  float eo = 1.0e-3f, w0 = 1.0f, w1 = 0.0f, w2 = 0.0f;
+  int found = 0;
   if (lam > 0.0 && ml > 0.0 && isfinite(lam) && isfinite(ml)) {
     double f = jrsq(ml);
     if (v0 < 0.0) f = -f;
     const double u0 = v0 * f;
     if (u0 < 100.0) {
-      *e_out = (float)lam;
-      u_out[0] = (float)u0;
-      u_out[1] = (float)(v1 * f);
-      u_out[2] = (float)(v2 * f);
-      u_out[3] = 0.0f;
-      return 1;
+      eo = (float)lam;
+      w0 = (float)u0;
+      w1 = (float)(v1 * f);
+      w2 = (float)(v2 * f);
+      found = 1;
     }
   }
-  *e_out = 1.0e-3f;
-  u_out[0] = 1.0f;
-  u_out[1] = u_out[2] = u_out[3] = 0.0f;
-  return 0;
+  *e_out = eo;
+  u_out[0] = w0;
+  u_out[1] = w1;
+  u_out[2] = w2;
+  u_out[3] = 0.0f;
+  return found;
 }
 
 __device__ float eos_pressure(float e, int eos) {
@@ -1173,14 +1178,15 @@ __device__ void cell_store_float(const EventLandau &ev, long is, const CellOut &
 // workgroup-uniform call.
 template <int N>
 __device__ void landau_cell_out(const EventLandau &ev, bool live, long is) {
+  // cell indices fit 32 bits (DIM is an int): 32-bit divisions
   long tidx = is;
   if (ev.t_eta_major) {
-    const long xy = is / ev.neta;
-    const long ie = is - xy * ev.neta;
-    tidx = ie * ev.nxy + xy;
+    const unsigned xy = (unsigned)is / (unsigned)ev.neta;
+    const unsigned ie = (unsigned)is - xy * (unsigned)ev.neta;
+    tidx = (long)ie * ev.nxy + xy;
   } else if (ev.t_row) {
-    const long ix = is / ev.t_ny;
-    tidx = ix * ev.t_row + (is - ix * ev.t_ny);
+    const unsigned ix = (unsigned)is / (unsigned)ev.t_ny;
+    tidx = (long)ix * ev.t_row + ((unsigned)is - ix * (unsigned)ev.t_ny);
   }
   CellOut r;
   landau_cell_compute<N>(ev, live, tidx, r);
