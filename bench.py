@@ -438,9 +438,12 @@ def main():
         "executed_mfma_frac": round(executed / FP16_MFMA_PEAK_TFLOPS, 4),
         "executed_mfma_flop_per_launch": mfma_flops,
         "mfma_per_event": info["mfma_batch"],
+        "algorithmic_hbm_bytes_per_launch": B * q1.DIM * (16 + 136),
         "note": "achieved = the reference's 52 FLOP per (cell, phi) sample x %d samples per launch (%d events) / "
                 "the launch's average time (HIP events on the launch stream), against MI355X's dense FP16 MFMA "
-                "peak; executed_mfma counts the v_mfma_f32_16x16x32_f16 instructions the kernel issues (%d per "
+                "peak; the launch also runs the fused 3x3 Landau stage in its tiles' epilogue and writes the 17 "
+                "output planes (16 B of knots in, 136 B out per cell: algorithmic_hbm_bytes_per_launch); "
+                "executed_mfma counts the v_mfma_f32_16x16x32_f16 instructions the kernel issues (%d per "
                 "event, 16384 FLOP each: knots and weights both split hi+lo, all four products, 12 of 16 columns used), %.1fx the algorithmic "
                 "work; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/)"
                 % (samples, B, info["mfma_batch"], mfma_flops / alg_flops),

@@ -1124,14 +1124,33 @@ void tmark(fsm_ctx *ctx, int i, hipStream_t s) {
   if (ctx->last >= 0) (void)hipEventRecord(ctx->pool[ctx->last][i], s);
 }
 
+EventLandau landau_event(const fsm_params &p, const float *T, int eta_major);  // below
+
+// The 2-D stream kernels' fused Landau stage: the tile epilogue solves the
+// 3x3 frame on the moments in registers and writes the 17 output planes of
+// d_out (events without a baryon current); EventLandau's constants.
+void fuse_landau2d(Event2D &ev, const fsm_params &p, double *d_out) {
+  const EventLandau L = landau_event(p, nullptr, 0);
+  ev.fuse_landau = 1;
+  ev.l_eos = L.eos_type;
+  ev.l_viscous = L.viscous;
+  ev.l_fill = L.fill_outputs;
+  ev.l_inv_tau2 = L.inv_tau2;
+  ev.l_out = d_out;
+  ev.l_out_plane = L.out_plane;
+}
+
 // Stage A into T with layout chosen by the plan (2-D: reference layout into
-// T_out; 3-D: eta-major into the plan's buffer).
+// T_out; 3-D: eta-major into the plan's buffer).  fuse_out (2-D only): run
+// the Landau stage in the stream kernel's epilogue into these outputs
+// instead of writing T.
 int stage_a(fsm_ctx *ctx, Plan &pl, const fsm_params &p, const float *d_e, float *T_ref, hipStream_t s,
-            const float **T_used, int *eta_major) {
+            const float **T_used, int *eta_major, double *fuse_out = nullptr) {
   if (!pl.is3d) {
     Event2D ev = pl.ev2;
     ev.e_gev = d_e;
     ev.T = T_ref;
+    if (fuse_out) fuse_landau2d(ev, p, fuse_out);
     if (ctx->tile_policy == FSM_TILES_BATCH) set_variant(ev, pl.bvariant[ctx->path]);
     if (ctx->path == FSM_PATH_MIXED) {
       ev.mfma = 1;
@@ -1255,8 +1274,14 @@ int run_device_impl(fsm_ctx *ctx, const fsm_params *p, const float *d_e, double 
   Plan *pl = plan.get();
   const float *T = nullptr;
   int em = 0;
-  st = stage_a(ctx, *pl, *p, d_e, T_scratch, s, &T, &em);
+  // 2-D events without a baryon current: Landau fused into the stream kernel
+  const bool fuse = !pl->is3d && !d_nb;
+  st = stage_a(ctx, *pl, *p, d_e, T_scratch, s, &T, &em, fuse ? d_out : nullptr);
   if (st) return st;
+  if (fuse) {
+    tmark(ctx, 3, s);
+    return FSM_OK;
+  }
   const float *J = nullptr;
   if (d_nb) {
     st = stream_charge(ctx, *pl, *p, d_nb, J_scratch, s, &J);
@@ -2258,6 +2283,12 @@ int batch_build(fsm_batch *b, hipStream_t s) {
     }
     int em = 0, jm = 0;
     const float *T = add_pass(ev, ev.d_e, false, cur, &em);
+    if (!em && !ev.d_nb) {
+      // a 2-D event without a baryon current: the Landau stage runs in its
+      // stream tiles' epilogue (no Landau items)
+      fuse_landau2d(e2.back(), p, ev.d_out);
+      continue;
+    }
     EventLandau L = landau_event(p, T, em);
     L.eta_zero = p.DIM_ETA == 1;  // the 2-D stream kernel writes the eta components as zeros
     if (!em) {
@@ -2490,10 +2521,10 @@ int fsm_run_device_noalloc(fsm_ctx *ctx, const fsm_params *p, const float *d_e_G
     const int v = ctx->tile_policy == FSM_TILES_BATCH ? pl.bvariant[ctx->path] : pl.ev2.variant;
     char *cur = ws;
     Event2D ev = event2d_in_workspace(pl, *p, d_e_GeV, ctx->path, v, cur, false);
+    fuse_landau2d(ev, *p, d_out);  // the Landau stage in the stream kernel's epilogue
     FSM_TRY(fsm_launch_spline2d(ev, s));
     FSM_TRY(fsm_launch_stream2d(ev, s));
-    L = landau_event(*p, ev.T, 0);
-    L.eta_zero = 1;
+    return FSM_OK;
   } else {
     // source planes and eta-major T in the workspace
     Event3D ev = pl.ev3;

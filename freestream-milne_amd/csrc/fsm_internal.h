@@ -103,6 +103,36 @@ __device__ __forceinline__ T fsm_sload(const T *p) {
   return __builtin_bit_cast(T, w);
 }
 
+// --------------------------------------------------------------- Landau ----
+#define FSM_NUM_OUTPUTS 17
+// 3-D tile of the Landau kernel: FSML_TE eta planes x FSML_TX transverse
+// cells (one cell per thread); outputs staged in LDS with a padded pitch.
+#define FSML_TE 16
+#define FSML_TX 16
+#define FSML_PITCH (FSML_TE + 1)
+#define FSML_PLANE (FSML_TX * FSML_PITCH)
+struct EventLandau {
+  int dim;
+  int nxy, neta;        // for the eta-major T layout
+  int t_eta_major;      // 1: T is eta-major (3-D stream output)
+  int eos_type;
+  int viscous;
+  int fill_outputs;     // OUTPUTFORMAT is 2 or 3
+  int eta_zero;         // 1: T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} are zero by construction (a 2-D
+                        // event's tensor from the stream kernel): the 3x3 Landau frame
+  float tau;
+  double inv_tau2;      // 1.0 / (double)(TAU * TAU), the per-event factor of pi^{eta eta} (src/LandauMatch.cpp:365)
+  const float *T;
+  long t_plane;
+  int t_ny, t_row;      // 2-D T with a padded row pitch: cell is = ix * t_ny + iy at ix * t_row + iy
+                        // (t_row = 0: T is in the reference layout or eta-major)
+  double *out;          // 17 planes of dim doubles (may be null)
+  long out_plane;
+  float *fe, *fu, *fp, *fPi, *fpi;  // optional float outputs
+  const float *J;       // baryon current J^tau, J^x, J^y, J^eta (T's layout and plane stride), or null
+  double *out_b;        // n_B, V^tau, V^x, V^y, V^eta: 5 planes of out_plane doubles (with J)
+};
+
 // ---------------------------------------------------------------- 2-D ----
 // Cells per lane along x in the 2-D stream kernel; one wave covers 64
 // consecutive iy.  Tile = (4 waves * FSM2D_M) x 64 cells.
@@ -212,6 +242,13 @@ struct Event2D {
   float jscale;         // DRAP of calculateBaryonCurrent (src/LandauMatch.cpp:140)
   int skip_eta_planes;  // 1: T is workspace read only by the 3x3 Landau frame, which does not read
                         // T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} (zeros): they are not written
+  // ---- fused Landau stage (events without a baryon current) ----
+  int fuse_landau;      // 1: the tile epilogue runs the 3x3 Landau frame on the moments in registers
+                        // and writes the 17 output planes (l_out); T is not written
+  int l_eos, l_viscous, l_fill;  // EventLandau::eos_type, viscous, fill_outputs
+  double l_inv_tau2;    // EventLandau::inv_tau2
+  double *l_out;        // EventLandau::out (17 planes of l_out_plane doubles)
+  long l_out_plane;
 };
 
 // The MFMA form of the 2-D quadrature.  For all samples j sharing a knot
@@ -306,36 +343,6 @@ struct Event3D {
   float norm_factor;    // energy: convertInitialDensity norm; charge pass: convertInitialChargeDensity norm
   int charge;           // 1: the input is n_B: src = n_B * norm_factor (no hbarc, no floor)
   int raw_e;            // 1: the energy input is already the fm^-4 density (IC_ENERGY != 5)
-};
-
-// --------------------------------------------------------------- Landau ----
-#define FSM_NUM_OUTPUTS 17
-// 3-D tile of the Landau kernel: FSML_TE eta planes x FSML_TX transverse
-// cells (one cell per thread); outputs staged in LDS with a padded pitch.
-#define FSML_TE 16
-#define FSML_TX 16
-#define FSML_PITCH (FSML_TE + 1)
-#define FSML_PLANE (FSML_TX * FSML_PITCH)
-struct EventLandau {
-  int dim;
-  int nxy, neta;        // for the eta-major T layout
-  int t_eta_major;      // 1: T is eta-major (3-D stream output)
-  int eos_type;
-  int viscous;
-  int fill_outputs;     // OUTPUTFORMAT is 2 or 3
-  int eta_zero;         // 1: T^{tau eta}, T^{x eta}, T^{y eta}, T^{eta eta} are zero by construction (a 2-D
-                        // event's tensor from the stream kernel): the 3x3 Landau frame
-  float tau;
-  double inv_tau2;      // 1.0 / (double)(TAU * TAU), the per-event factor of pi^{eta eta} (src/LandauMatch.cpp:365)
-  const float *T;
-  long t_plane;
-  int t_ny, t_row;      // 2-D T with a padded row pitch: cell is = ix * t_ny + iy at ix * t_row + iy
-                        // (t_row = 0: T is in the reference layout or eta-major)
-  double *out;          // 17 planes of dim doubles (may be null)
-  long out_plane;
-  float *fe, *fu, *fp, *fPi, *fpi;  // optional float outputs
-  const float *J;       // baryon current J^tau, J^x, J^y, J^eta (T's layout and plane stride), or null
-  double *out_b;        // n_B, V^tau, V^x, V^y, V^eta: 5 planes of out_plane doubles (with J)
 };
 
 #endif  // FSM_INTERNAL_H_

@@ -28,6 +28,28 @@
 
 namespace {
 
+// the 3x3 Landau frame and outputs of the fused epilogue (Event2D::fuse_landau)
+#include "fsm_landau.h"
+
+// The same as a call with scalar arguments, for the FP32 tiles that an MFMA
+// kernel runs where the spline can undershoot (rare): inlined there too, the
+// MFMA kernel's code would exceed its register cap of 168 and spill.
+__device__ __noinline__ void landau2d_store_call(double *out, long out_plane, int eos, int viscous, int fill,
+                                                 float tau, double inv_tau2, long is, float t0, float t1, float t2,
+                                                 float t3, float t4, float t5) {
+  Event2D s;
+  s.l_out = out;
+  s.l_out_plane = out_plane;
+  s.l_eos = eos;
+  s.l_viscous = viscous;
+  s.l_fill = fill;
+  s.tau = tau;
+  s.l_inv_tau2 = inv_tau2;
+  const float t6[6] = {t0, t1, t2, t3, t4, t5};
+  landau2d_cell_store(s, t6, is);
+}
+
+
 // Copies the knot box [bx, bx+LX) x [by, by+LY) of an (nx, ny) x-major plane
 // into LDS (zero outside the grid), column pitch `pitch` >= LY elements.  Each
 // thread keeps U loads in flight and walks the box with an incremental (row,
@@ -108,7 +130,7 @@ __device__ __forceinline__ bool ref_off_spline(const Event2D &ev, const Stencil2
   return true;
 }
 
-template <int M, int JS>
+template <int M, int JS, bool OUTLINE_LANDAU = false>
 __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, float4 *lds) {
   constexpr int CG = 4 / JS;          // cell groups per workgroup
   constexpr int TX = CG * 2 * M;      // tile width in x
@@ -229,6 +251,37 @@ __device__ void stream2d_tile_v2(const Event2D &ev, int tile_x, int tile_y, floa
     }
   }
   if (js != 0) return;  // after the last barrier; the one-event kernels have no loop
+  if (ev.fuse_landau) {
+    // Fused Landau stage: the lane's M cells one at a time, their moments
+    // staged in the lane's own LDS slot (M * 5 floats: the slot the JS
+    // reduction gives this wave, which no other wave reads), so that the
+    // solve of one cell is not interleaved with the others' (registers).
+    float *slot = reinterpret_cast<float *>(lds) + (wave * 64 + lane) * (M * 5);
+    if (JS == 1) __syncthreads();  // every wave is done with the last group's knot box
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) slot[m * 5 + c] = acc[m][c];
+#pragma unroll 1
+    for (int m = 0; m < M; ++m) {
+      const int ix = ixb + m;
+      if (ix >= nx || iy >= ny) continue;
+      float a[5], r[6];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) a[c] = slot[m * 5 + c];
+      const float s5 = a[0] - a[3];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) r[c] = ((a[c] * ev.out_scale) * ev.d_phip) / ev.tau;
+      r[5] = ((s5 * ev.out_scale) * ev.d_phip) / ev.tau;
+      // the Landau stage on these floats
+      if (OUTLINE_LANDAU)
+        landau2d_store_call(ev.l_out, ev.l_out_plane, ev.l_eos, ev.l_viscous, ev.l_fill, ev.tau, ev.l_inv_tau2,
+                            (long)ix * ny + iy, r[0], r[1], r[2], r[3], r[4], r[5]);
+      else
+        landau2d_cell_store(ev, r, (long)ix * ny + iy);
+    }
+    return;
+  }
   // src/LandauMatch.cpp:104: T = sum * d_phip / TAU.  The s^2 row is
   // sum(v) - sum(v c^2) (c^2 + s^2 = 1); rows 3, 6, 8, 9 carry tanh(0) = 0.
   const long P = ev.t_plane;
@@ -535,7 +588,7 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     for (int k = tid; k < cnt; k += blockDim.x)
       bad |= fsm_gload(ev.sp_flags + (sx0 + k / cy) * ev.sp_tiles_y + sy0 + k % cy) != 0u;
     if (__syncthreads_or(bad)) {
-      stream2d_tile_v2<MfmaShape<V>::M, MfmaShape<V>::JS>(ev, tile_x, tile_y, lds);
+      stream2d_tile_v2<MfmaShape<V>::M, MfmaShape<V>::JS, true>(ev, tile_x, tile_y, lds);
       return;
     }
   }
@@ -640,6 +693,10 @@ __device__ void stream2d_mfma_tile(const Event2D &ev, int tile_x, int tile_y, fl
     float rr[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) rr[m] = ((t[q][m] * ev.out_scale) * ev.d_phip) / ev.tau;
+    if (ev.fuse_landau) {
+      landau2d_cell_store(ev, rr, (long)cell_x(q) * ny + cell_y(q));  // the Landau stage on these floats
+      continue;
+    }
     const long is = (long)cell_x(q) * ev.t_row + cell_y(q);
     fsm_gstore(ev.T + 0 * P + is, rr[0]);
     fsm_gstore(ev.T + 1 * P + is, rr[1]);
@@ -721,8 +778,8 @@ int fsm_stream2d_tile_h(int) { return 32; }
 
 size_t fsm_stream2d_lds_bytes(int variant, int box_floats4) {
   size_t box = (size_t)box_floats4 * sizeof(float4);
-  size_t red = 0;
-  if (variant == 1) red = (size_t)256 * 4 * 5 * sizeof(float);
+  // the JS reduction, and the fused Landau epilogue's per-lane slots: M * 5 floats per thread
+  size_t red = (size_t)256 * 4 * 5 * sizeof(float);
   if (variant == 2) red = (size_t)256 * 2 * 5 * sizeof(float);
   return box > red ? box : red;
 }
