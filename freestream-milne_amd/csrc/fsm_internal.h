@@ -77,6 +77,16 @@ __device__ __forceinline__ uint2 fsm_gload(const uint2 *p) {
 __device__ __forceinline__ unsigned fsm_gload(const unsigned *p) { return *(const FSM_AS_GLOBAL unsigned *)p; }
 __device__ __forceinline__ float fsm_gload(const float *p) { return *(const FSM_AS_GLOBAL float *)p; }
 __device__ __forceinline__ void fsm_gstore(float *p, float v) { *(FSM_AS_GLOBAL float *)p = v; }
+// Element idx of a plane whose base is wave-uniform (kept in scalar
+// registers): a global store with a 32-bit per-lane byte offset, no per-lane
+// 64-bit address arithmetic.  The byte offset is formed in 32 bits, so the
+// plane holds fewer than 2^29 doubles (2^30 floats): a cell index, far below.
+__device__ __forceinline__ void fsm_gstore_at(double *base, unsigned idx, double v) {
+  *(FSM_AS_GLOBAL double *)((FSM_AS_GLOBAL char *)base + (idx << 3)) = v;
+}
+__device__ __forceinline__ void fsm_gstore_at(float *base, unsigned idx, float v) {
+  *(FSM_AS_GLOBAL float *)((FSM_AS_GLOBAL char *)base + (idx << 2)) = v;
+}
 __device__ __forceinline__ void fsm_gstore(uint4 *p, uint4 v) {
   *(FSM_AS_GLOBAL fsm_v4u *)p = fsm_v4u{v.x, v.y, v.z, v.w};
 }

@@ -746,12 +746,13 @@ __device__ void landau_cell_out(const EventLandau &ev, bool live, long is) {
   if (!live) return;
   if (ev.out) {
 #pragma unroll
-    for (int c = 0; c < FSM_NUM_OUTPUTS; ++c) ev.out[c * ev.out_plane + is] = plane_double(ev, cell_value(r, c), c);
+    for (int c = 0; c < FSM_NUM_OUTPUTS; ++c)
+      fsm_gstore_at(ev.out + c * ev.out_plane, (unsigned)is, plane_double(ev, cell_value(r, c), c));
   }
   if (ev.J) {
 #pragma unroll
     for (int c = FSM_NUM_OUTPUTS; c < FSM_NUM_OUTPUTS + 5; ++c)
-      ev.out_b[(c - FSM_NUM_OUTPUTS) * ev.out_plane + is] = plane_double(ev, cell_value(r, c), c);
+      fsm_gstore_at(ev.out_b + (c - FSM_NUM_OUTPUTS) * ev.out_plane, (unsigned)is, plane_double(ev, cell_value(r, c), c));
   }
   cell_store_float(ev, is, r);
 }
@@ -786,7 +787,7 @@ __device__ void landau3d_tile(const EventLandau &ev, int xt, int et, float *lds)
     const int li = wxl * FSML_PITCH + wel;
     for (int c = (ev.out ? 0 : FSM_NUM_OUTPUTS); c < np; ++c) {
       double *o = c < FSM_NUM_OUTPUTS ? ev.out + c * ev.out_plane : ev.out_b + (c - FSM_NUM_OUTPUTS) * ev.out_plane;
-      o[is] = plane_double(ev, lds[c * FSML_PLANE + li], c);
+      fsm_gstore_at(o, (unsigned)is, plane_double(ev, lds[c * FSML_PLANE + li], c));
     }
   }
   __syncthreads();  // every lane has read its staged outputs

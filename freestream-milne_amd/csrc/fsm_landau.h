@@ -451,16 +451,17 @@ __device__ __forceinline__ double plane_double(const EventLandau &ev, float x, i
 // Float outputs of the function-level boundary (fsm_landau_device).
 __device__ void cell_store_float(const EventLandau &ev, long is, const CellOut &r) {
   const long D = ev.dim;
-  if (ev.fe) ev.fe[is] = r.e;
+  const unsigned i = (unsigned)is;
+  if (ev.fe) fsm_gstore_at(ev.fe, i, r.e);
   if (ev.fu) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) ev.fu[c * D + is] = r.u[c];
+    for (int c = 0; c < 4; ++c) fsm_gstore_at(ev.fu + c * D, i, r.u[c]);
   }
-  if (ev.fp) ev.fp[is] = r.p;
-  if (ev.fPi) ev.fPi[is] = r.Pi;
+  if (ev.fp) fsm_gstore_at(ev.fp, i, r.p);
+  if (ev.fPi) fsm_gstore_at(ev.fPi, i, r.Pi);
   if (ev.fpi) {
 #pragma unroll
-    for (int c = 0; c < 10; ++c) ev.fpi[c * D + is] = r.pi[c];
+    for (int c = 0; c < 10; ++c) fsm_gstore_at(ev.fpi + c * D, i, r.pi[c]);
   }
 }
 
@@ -486,7 +487,8 @@ __device__ __forceinline__ void landau2d_cell_store(const Event2D &s, const floa
   CellOut r;
   landau_cell_from_t<3>(ev, t, 0, r);
 #pragma unroll
-  for (int c = 0; c < FSM_NUM_OUTPUTS; ++c) s.l_out[c * s.l_out_plane + is] = plane_double(ev, cell_value(r, c), c);
+  for (int c = 0; c < FSM_NUM_OUTPUTS; ++c)
+    fsm_gstore_at(s.l_out + c * s.l_out_plane, (unsigned)is, plane_double(ev, cell_value(r, c), c));
 }
 
 #endif  // FSM_LANDAU_H_
