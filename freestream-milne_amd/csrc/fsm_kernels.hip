@@ -218,12 +218,18 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
     const int gy = y0 - B + lane;
     const bool yin = gy >= 0 && gy < ny;
     float z[NS];
+    if (ev.zk) {  // converted by the knot-scale pass
 #pragma unroll
-    for (int u = 0; u < NS; ++u) {
-      const int gx = x0 - B + wave + 4 * u;
-      z[u] = (yin && gx >= 0 && gx < nx)
-                 ? knot_z(ev, fsm_gload(ev.e_gev + (long)gx * ny + gy))
-                 : 0.0f;
+      for (int u = 0; u < NS; ++u) {
+        const int gx = x0 - B + wave + 4 * u;
+        z[u] = (yin && gx >= 0 && gx < nx) ? fsm_gload(ev.zk + (long)gx * ny + gy) : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+        const int gx = x0 - B + wave + 4 * u;
+        z[u] = (yin && gx >= 0 && gx < nx) ? knot_z(ev, fsm_gload(ev.e_gev + (long)gx * ny + gy)) : 0.0f;
+      }
     }
     // largest |z| over each flag tile's band region (staged rows [16 i, 16 i + 47])
     float zm[NF];
@@ -373,13 +379,24 @@ __device__ void knot_scale_part(const Event2D &ev, int part, float *red) {
   float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
   const long step = blockDim.x;
   long k = k0 + threadIdx.x;
+  float *zk = ev.zk;  // the knot values, kept for the batch spline kernel
   for (; k + 3 * step < k1; k += 4 * step) {
-    m0 = fmaxf(m0, fabsf(knot_z(ev, ev.e_gev[k])));
-    m1 = fmaxf(m1, fabsf(knot_z(ev, ev.e_gev[k + step])));
-    m2 = fmaxf(m2, fabsf(knot_z(ev, ev.e_gev[k + 2 * step])));
-    m3 = fmaxf(m3, fabsf(knot_z(ev, ev.e_gev[k + 3 * step])));
+    float z[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) z[j] = knot_z(ev, fsm_gload(ev.e_gev + k + j * step));
+    if (zk)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) fsm_gstore_at(zk, (unsigned)(k + j * step), z[j]);
+    m0 = fmaxf(m0, fabsf(z[0]));
+    m1 = fmaxf(m1, fabsf(z[1]));
+    m2 = fmaxf(m2, fabsf(z[2]));
+    m3 = fmaxf(m3, fabsf(z[3]));
   }
-  for (; k < k1; k += step) m0 = fmaxf(m0, fabsf(knot_z(ev, ev.e_gev[k])));
+  for (; k < k1; k += step) {
+    const float z = knot_z(ev, fsm_gload(ev.e_gev + k));
+    if (zk) fsm_gstore_at(zk, (unsigned)k, z);
+    m0 = fmaxf(m0, fabsf(z));
+  }
   float m = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
   __syncthreads();
