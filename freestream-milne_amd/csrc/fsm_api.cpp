@@ -2043,10 +2043,11 @@ Event2D event2d_in_workspace(const Plan &pl, const fsm_params &p, const float *d
   return d;
 }
 
-// Batches on the MFMA path: the knot-scale pass writes the knot values z
-// into the first plane of the event's T, which the batch spline kernel reads
-// (each input value converted once, not once per tile halo that holds it);
-// the stream kernel, launched after the spline kernel, overwrites that plane.
+// Batches: the knot pass writes the knot values z into the first plane of
+// the event's T, which the batch spline kernel reads (each input value
+// converted once, not once per tile halo that holds it); the stream kernel,
+// launched after the spline kernel, overwrites that plane.  The pass also
+// writes the partial maxima of |z| that the MFMA path's knot scale needs.
 inline void keep_knots_in_t(Event2D &d) { d.zk = d.T; }
 
 // Autotuner of the batch tile shape: times the 2-D stream kernel of this
@@ -2087,10 +2088,8 @@ int autotune2d(fsm_ctx *ctx, Plan &pl, const fsm_params &p) {
     const int sty = (ny + FSMSP_TY - 1) / FSMSP_TY;
     for (int i = 0; i < n; ++i) {
       Event2D d = event2d_in_workspace(pl, p, d_in.p, path, v, cur, true);
-      if (d.mfma) {
-        keep_knots_in_t(d);
-        for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
-      }
+      keep_knots_in_t(d);
+      for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
       for (int x = 0; x < (nx + FSMSP2_TX - 1) / FSMSP2_TX; ++x)
         for (int y = 0; y < sty; ++y) sp.push_back(make_int4(i, x, y, 0));
       for (int y = 0; y < d.tiles_y; ++y)
@@ -2248,10 +2247,8 @@ int batch_build(fsm_batch *b, hipStream_t s) {
         d.out_scale = 1.0f;  // DIM_RAP = 1
       }
       const int i = (int)e2.size();
-      if (d.mfma) {
-        keep_knots_in_t(d);
-        for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
-      }
+      keep_knots_in_t(d);
+      for (int part = 0; part < FSM_KS_PARTS; ++part) ks.push_back(make_int4(i, part, 0, 0));
       e2.push_back(d);
       const int sty = (p.DIM_Y + FSMSP_TY - 1) / FSMSP_TY;
       for (int x = 0; x < (p.DIM_X + FSMSP2_TX - 1) / FSMSP2_TX; ++x)

@@ -218,8 +218,8 @@ struct Event2D {
   float xmin, ymin;     // grid origin as getX/getY compute it
   float dx, dy;
   const float *e_gev;   // input, GeV/fm^3
-  float *zk;            // batches, MFMA path: knot values z of e_gev (written by the knot-scale pass, read by
-                        // the batch spline kernel); nullptr: the spline kernel converts e_gev itself
+  float *zk;            // batches: knot values z of e_gev (written by the knot pass, read by the batch
+                        // spline kernel); the one-event kernels convert e_gev themselves
   float4 *planes;       // spline planes (written by the spline kernel)
   const float *gx;      // [nx][FSM_SPLINE_W] derivative band weights along x
   const float *gy;      // [ny][FSM_SPLINE_W]

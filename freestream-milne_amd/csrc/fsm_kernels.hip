@@ -177,79 +177,62 @@ __device__ void spline2d_tile(const Event2D &ev, int tx, int ty, SplineLds &L) {
 // sequential fmaf chain over the same band weights, and each FSMSP_TX x
 // FSMSP_TY flag tile keeps its own positivity tolerance) but with the band
 // products from registers: for z_x a lane owns one column and 16 rows and
-// slides a 48-value window down the column; for z_y and z_xy a lane owns one
-// row and 8 columns and slides a 40-value window along the row.  A wave's
-// rows (z_x) or columns (z_y) are uniform, so the weights are scalar loads,
-// one set for the whole wave where its rows or columns all carry the
-// interior weights ([cxlo, cxhi], [cylo, cyhi]).
-#define SP2_RX (FSMSP2_TX + 2 * FSM_SPLINE_BAND)
+// slides a 48-value window down the column, read from the knot values the
+// knot pass wrote (zk); for z_y and z_xy a lane owns one row and 8 columns and
+// slides a 40-value window along the row, from LDS.  A wave's rows (z_x) or
+// columns (z_y) are uniform, so the weights are scalar loads, one set for the
+// whole wave where its rows or columns all carry the interior weights
+// ([cxlo, cxhi], [cylo, cyhi]).  Only the tile's own rows are staged in LDS
+// (33.8 KiB per workgroup, four per CU).
 #define SP2_RY (FSMSP_TY + 2 * FSM_SPLINE_BAND)
 #define SP2_P (SP2_RY + 1)  // odd pitch: lanes walking rows do not share banks
 #define SP2_OP (FSMSP_TY + 1)  // staged knots per tile row (float4): rows start in different banks
+#define SP2_ZBUF (FSMSP2_TX * SP2_OP * 4 > 2 * FSMSP2_TX * SP2_P ? FSMSP2_TX * SP2_OP * 4 : 2 * FSMSP2_TX * SP2_P)
 // The band weights are read-only for the kernel's lifetime: read through the
 // constant address space, so uniform reads become scalar loads.
 typedef __attribute__((address_space(4))) const float fsm_const_f;
 
 struct __align__(16) SplineLds2 {
-  // z over the tile and the band-wide halo (zr), then z_x over the tile rows
-  // and the y halo (zxr); afterwards the staged knots of the tile
-  float zbuf[SP2_RX * SP2_P + FSMSP2_TX * SP2_P];
+  // z over the tile rows and the y halo (zr), then z_x over the same (zxr);
+  // afterwards the staged knots of the tile
+  float zbuf[SP2_ZBUF];
   __device__ float *zr() { return zbuf; }
-  __device__ float *zxr() { return zbuf + SP2_RX * SP2_P; }
-  float zmax[4][FSMSP2_TX / FSMSP_TX];
+  __device__ float *zxr() { return zbuf + FSMSP2_TX * SP2_P; }
+  float zmax[FSMSP2_TX / FSMSP_TX];  // largest |z| over each flag tile's band region
   unsigned bad[FSMSP2_TX / FSMSP_TX];
   float sc;
 };
-static_assert(FSMSP2_TX * SP2_OP * 4 <= SP2_RX * SP2_P + FSMSP2_TX * SP2_P, "the staged knots fit in zbuf");
 
 __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L) {
   constexpr int B = FSM_SPLINE_BAND, W = FSM_SPLINE_W, NF = FSMSP2_TX / FSMSP_TX;
   constexpr int RA = FSMSP2_TX / 4;       // z_x rows per wave
   constexpr int CB = FSMSP_TY / 4;        // z_y columns per wave
-  constexpr int NS = SP2_RX / 4;          // staged rows per wave
   static_assert(SP2_RY == 64 && FSMSP2_TX == 64, "one lane per staged column and per tile row");
+  static_assert(RA == FSMSP_TX, "a wave's z_x rows are one flag tile, its window that tile's band region");
   const int nx = ev.nx, ny = ev.ny;
   const int x0 = tx2 * FSMSP2_TX, y0 = ty * FSMSP_TY;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  __syncthreads();  // the previous item's reads of L are done
-  // z over staged rows s = wave + 4 u (x = x0 - B + s), lane = column (y = y0 - B + lane)
+  // z_x (grid units) for tile rows x0 + RA wave + r and staged column lane
+  // (y = y0 - B + lane), from the column's z over rows x0 + RA wave - B ...
+  // x0 + RA wave + RA + B - 1
   {
     const int gy = y0 - B + lane;
     const bool yin = gy >= 0 && gy < ny;
-    float z[NS];
-    if (ev.zk) {  // converted by the knot-scale pass
+    const int r0 = x0 + RA * wave;
+    float win[RA + 2 * B];
 #pragma unroll
-      for (int u = 0; u < NS; ++u) {
-        const int gx = x0 - B + wave + 4 * u;
-        z[u] = (yin && gx >= 0 && gx < nx) ? fsm_gload(ev.zk + (long)gx * ny + gy) : 0.0f;
-      }
-    } else {
-#pragma unroll
-      for (int u = 0; u < NS; ++u) {
-        const int gx = x0 - B + wave + 4 * u;
-        z[u] = (yin && gx >= 0 && gx < nx) ? knot_z(ev, fsm_gload(ev.e_gev + (long)gx * ny + gy)) : 0.0f;
-      }
+    for (int u = 0; u < RA + 2 * B; ++u) {
+      const int gx = r0 - B + u;
+      win[u] = (yin && gx >= 0 && gx < nx) ? fsm_gload(ev.zk + (long)gx * ny + gy) : 0.0f;
     }
-    // largest |z| over each flag tile's band region (staged rows [16 i, 16 i + 47])
-    float zm[NF];
-#pragma unroll
-    for (int i = 0; i < NF; ++i) zm[i] = 0.0f;
-#pragma unroll
-    for (int u = 0; u < NS; ++u) {
-      const int srow = wave + 4 * u;
-      L.zr()[srow * SP2_P + lane] = z[u];
-#pragma unroll
-      for (int i = 0; i < NF; ++i)
-        if (srow >= FSMSP_TX * i && srow < FSMSP_TX * i + FSMSP_TX + 2 * B) zm[i] = fmaxf(zm[i], fabsf(z[u]));
-    }
+    __syncthreads();  // the previous item's reads of L are done
     if (ev.planes_h) {
+      float m = 0.0f;
 #pragma unroll
-      for (int i = 0; i < NF; ++i) {
-        float m = zm[i];
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if (lane == 0) L.zmax[wave][i] = m;
-      }
+      for (int u = 0; u < RA + 2 * B; ++u) m = fmaxf(m, fabsf(win[u]));
+      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+      if (lane == 0) L.zmax[wave] = m;
       if (threadIdx.x < 64) {
         const float sc = knot_scale_from_parts(ev);
         if (threadIdx.x == 0) {
@@ -259,16 +242,8 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
       }
       if (threadIdx.x < NF) L.bad[threadIdx.x] = 0u;
     }
-  }
-  __syncthreads();
-  // z_x (grid units) for tile rows x0 + RA wave + r and staged column lane
-  {
-    const int gy = y0 - B + lane;
-    const bool yin = gy >= 0 && gy < ny;
-    float win[RA + 2 * B];
 #pragma unroll
-    for (int u = 0; u < RA + 2 * B; ++u) win[u] = L.zr()[(RA * wave + u) * SP2_P + lane];
-    const int r0 = x0 + RA * wave;
+    for (int r = 0; r < RA; ++r) L.zr()[(RA * wave + r) * SP2_P + lane] = win[r + B];
     if (r0 >= ev.cxlo && r0 + RA - 1 <= ev.cxhi) {
       fsm_const_f *w = (fsm_const_f *)ev.gx + (long)ev.cxlo * W;  // every row of this wave has these weights
 #pragma unroll
@@ -299,7 +274,7 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
     float zw[CB + 2 * B], zxw[CB + 2 * B];
 #pragma unroll
     for (int u = 0; u < CB + 2 * B; ++u) {
-      zw[u] = L.zr()[(lane + B) * SP2_P + CB * wave + u];
+      zw[u] = L.zr()[lane * SP2_P + CB * wave + u];
       zxw[u] = L.zxr()[lane * SP2_P + CB * wave + u];
     }
     const int c0 = y0 + CB * wave;
@@ -337,14 +312,7 @@ __device__ void spline2d_tile2(const Event2D &ev, int tx2, int ty, SplineLds2 &L
         const float4 v = stg[row * SP2_OP + col];
         fsm_gstore(ev.planes + (long)gx * ny + gy, v);
         if (ev.planes_h) {
-          const int fi = row / FSMSP_TX;
-          float zmx = 0.0f;
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int i = 0; i < NF; ++i)
-              if (i == fi) zmx = fmaxf(zmx, L.zmax[u][i]);
-          const float tol = -FSM_MFMA_POS_EPS * zmx;
+          const float tol = -FSM_MFMA_POS_EPS * L.zmax[row / FSMSP_TX];
           const float big = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))) * sc;
           bad = !(knot_cp_min(v.x, v.y, v.z, v.w) >= tol) || !(big <= FSM_MFMA_MAXABS);
           unsigned short h[4], l[4];
@@ -427,14 +395,16 @@ __global__ __launch_bounds__(256) void k_knot_scale_one(Event2D ev) {
   knot_scale_part(ev, blockIdx.x, red);
 }
 
-// Batched: item = (event, part).
+// Batched: item = (event, part), for every 2-D event of a batch (the knot
+// values go to zk for the batch spline kernel; the maxima are used on the
+// MFMA path only).
 __global__ __launch_bounds__(256) void k_knot_scale_batch(const Event2D *__restrict__ evs,
                                                           const int4 *__restrict__ items, int n_items) {
   __shared__ float red[4];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int4 item = fsm_sload(items + xcd_order(it, n_items));
     const Event2D ev = fsm_sload(evs + item.x);
-    if (ev.planes_h) knot_scale_part(ev, item.y, red);
+    knot_scale_part(ev, item.y, red);
     __syncthreads();
   }
 }
