@@ -30,6 +30,31 @@ def test_batch_matches_single_events_over_graph_replays(gpu_ctx):
         gpu_ctx.set_tile_policy(fsm_amd.TILES_AUTO)
 
 
+def _small_2d_events():
+    """2-D grids smaller than one spline tile (16 x 32 knots) or one stream
+    tile, with rings of 1.7 to 4 cells: the batch spline kernel's bands and
+    windows reach past every edge of the grid."""
+    evs = []
+    for nx, ny, dx, dtau, nphi, seed in ((21, 23, 0.1, 0.4, 24, 21), (9, 13, 0.3, 0.5, 16, 22), (17, 40, 0.12, 0.3, 20, 23)):
+        p = W.base_params(DIM_X=nx, DIM_Y=ny, DIM_PHIP=nphi, DX=dx, DY=dx, TAU0=0.3, DTAU=dtau)
+        e = W.hotspots_2d(nx, ny, dx, dx, 12, seed, width=0.4).ravel()
+        evs.append((fsm_amd.params_finalize(p, e), e))
+    return evs
+
+
+@pytest.mark.parametrize("path", ["fp32", "mixed"])
+def test_batch_small_grids_match_single_events(gpu_ctx, path):
+    evs = _small_2d_events()
+    gpu_ctx.set_path(fsm_amd.PATH_MIXED if path == "mixed" else fsm_amd.PATH_FP32)
+    gpu_ctx.set_tile_policy(fsm_amd.TILES_BATCH)
+    try:
+        _batch_vs_single(gpu_ctx, evs)
+    finally:
+        # the session's shared context goes back to its defaults
+        gpu_ctx.set_tile_policy(fsm_amd.TILES_AUTO)
+        gpu_ctx.set_path(fsm_amd.PATH_FP32)
+
+
 def _batch_vs_single(gpu_ctx, evs):
     import torch
 
