@@ -60,6 +60,10 @@ def _case(name):
         from test_gpu_parity import reference_trento_event
 
         return reference_trento_event()
+    if name in ("tiny2d", "small2d"):  # grids smaller than one spline or stream tile
+        from test_gpu_parity import _case as parity_case
+
+        return parity_case(name)
     if name == "int2d":  # DTAU/DX = 10: samples on grid lines next to the mask edge
         p = W.base_params(DIM_X=101, DIM_Y=101, DIM_PHIP=64, DX=0.08, DY=0.08, TAU0=0.1, DTAU=0.8)
         return p, W.hotspots_2d(101, 101, 0.08, 0.08, 60, 13).ravel()
@@ -85,7 +89,7 @@ def _gpu(ctx, q, e):
 
 
 @pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "trento_ref", "small3d",
-                                  "small3d_int", "rect3d", "wide3d", "lastline2d"])
+                                  "small3d_int", "rect3d", "wide3d", "lastline2d", "tiny2d", "small2d"])
 def test_mixed_parity(mixed_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)

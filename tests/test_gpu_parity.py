@@ -23,6 +23,11 @@ def _case(name):
     if name == "rect2d":  # DIM_X != DIM_Y, DX != DY, two identical rapidity copies
         p = W.base_params(DIM_X=77, DIM_Y=64, DIM_RAP=2, DIM_PHIP=40, DX=0.08, DY=0.11, TAU0=0.2, DTAU=0.7)
         return p, W.hotspots_2d(77, 64, 0.08, 0.11, 30, 11).ravel()
+    if name in ("tiny2d", "small2d"):  # grids smaller than one spline or stream tile: every cell near an edge
+        from test_gpu_batch import _small_2d_events
+
+        q, e = _small_2d_events()[1 if name == "tiny2d" else 0]
+        return q, e
     if name == "int2d":  # DTAU/DX = 10: samples on grid lines next to the mask edge; hotspots at the edges
         p = W.base_params(DIM_X=101, DIM_Y=101, DIM_PHIP=64, DX=0.08, DY=0.08, TAU0=0.1, DTAU=0.8)
         return p, W.hotspots_2d(101, 101, 0.08, 0.08, 60, 13).ravel()
@@ -83,7 +88,7 @@ def _gpu(ctx, q, e):
 
 
 @pytest.mark.parametrize("name", ["C0", "C1", "rect2d", "int2d", "spiky", "hotspots_spikes", "small3d", "small3d_int",
-                                  "trento_ref", "lastline2d"])
+                                  "trento_ref", "lastline2d", "tiny2d", "small2d"])
 def test_parity_against_reference_float_path(gpu_ctx, name):
     p, e = _case(name)
     q = fsm_amd.params_finalize(p, e)
