@@ -331,8 +331,8 @@ def c3_run(fsm_amd, W, ctx, stream, steps, n_events, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(events shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path here "
+              "(events would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
     if args.profile_3d:
         profile_3d()

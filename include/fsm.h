@@ -412,7 +412,7 @@ int fsm_server_stats(fsm_server *s, int64_t *admitted, int64_t *completed, int64
                      size_t *bytes_in_use);
 
 /* ------------------------------------------------------------------ */
-/* Input adapters (SURVEY next-row f2): the reference's initial conditions and
+/* Input adapters: the reference's initial conditions and
  * profile readers, as host functions that fill a DIM-float vector in cell
  * order.  A vector made here is the input of fsm_run / fsm_run_device with
  * the same params: for IC_ENERGY != 5 the library takes the input as the
@@ -449,7 +449,7 @@ enum {
 int fsm_read_profile(const fsm_params *p, int format, const char *path, float *out, size_t n);
 
 /* ------------------------------------------------------------------ */
-/* OUTPUTFORMAT=1 text writers (SURVEY next-row f3), byte-for-byte the
+/* OUTPUTFORMAT=1 text writers, byte-for-byte the
  * reference's: writeScalarToFile / writeVectorToFile (src/FileIO.cpp:11-81,
  * "x y eta value" per cell) and the eta = 0 projections
  * (writeScalarToFileProjection, :85-131).  `path` is the file to write (the
@@ -464,7 +464,7 @@ int fsm_energy_sums(const fsm_params *p, const float *T00, const float *e, float
 int fsm_write_energy_inside(const char *path, float inside);
 
 /* ------------------------------------------------------------------ */
-/* Diagnostics (SURVEY next-row f4) on the device, from the float hydro
+/* Diagnostics on the device, from the float hydro
  * fields of fsm_landau_device (all device pointers, DIM floats per plane;
  * u has 4 planes, pi 10).  Outputs (any may be NULL):
  *   R_Pi_Inv, R_pimunu_Inv   calculateBulkInvReynolds / calculateShearInvReynolds

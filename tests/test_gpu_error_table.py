@@ -1,7 +1,7 @@
 """Every output's error against the FP64 oracle, for both GPU paths and for
 the reference's own float path (the oracle's float instantiation), on the
 261^2 lumpy event (C1), the 101^2 Gaussian (C0) and a reduced 3-D event.  The table is printed
-(run with -s) and recorded in DESIGN.md.
+(run with -s).
 
 Scales (as in tests/parity.py): T^{mu nu} relative to max T^{tau tau}; e and
 p relative to each cell's value; u^mu relative to the cell's u^tau; pi^{mu nu}
